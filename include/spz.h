@@ -1,0 +1,148 @@
+/*
+ * spz.h -- C ABI of libspz, the MI355X IDA* solver for the 15- and 24-puzzle.
+ *
+ * The reference has no library API: every solver is a process that reads
+ * problem files and prints "<path> <len> [nodes]" (src/24puzzle.cc:167-198,
+ * src_horie/idas_cpu.c:370-389).  These entry points replace that solver
+ * surface; each cites the reference code whose behaviour it takes over.
+ *
+ * Boards: row-major uint8 arrays of width*width cells, 0 = blank, goal has
+ * tile t at cell t (src/24puzzle.cc:89-108).  Node counts follow the
+ * reference: every in-bounds child other than the reverse of the previous
+ * move is counted when its parent is expanded (src/24puzzle.cc:119-131);
+ * thresholds are h0, h0+2, ... (src/24puzzle.cc:146-152).  Counts are u64.
+ *
+ * Errors: functions return SPZ_OK (0) or a negative SPZ_E* code and never
+ * exit or trap (the reference exits on CUDA errors,
+ * src/15puzzle_block_parallel.cu:42-57).  spz_last_error() describes the
+ * last failure.  A context is not thread-safe; use one per host thread.
+ */
+#ifndef SPZ_H
+#define SPZ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPZ_OK 0
+#define SPZ_E_ARG -1      /* bad argument (non-permutation board, sizes) */
+#define SPZ_E_HIP -2      /* HIP runtime error */
+#define SPZ_E_NOPDB -3    /* PDB heuristic requested before spz_build_pdb */
+#define SPZ_E_CAPACITY -4 /* device work queue full (raise queue_items) */
+#define SPZ_E_IO -5       /* file read/write failed */
+#define SPZ_E_UNSOLVABLE -6
+#define SPZ_E_STALL -7    /* internal: work vanished with instances active */
+#define SPZ_E_TIMEOUT -8  /* wall-clock limit (env SPZ_TIMEOUT_S, default 900) */
+
+/* heuristics */
+#define SPZ_H_MANHATTAN 0 /* src/24puzzle.cc:89-108 */
+#define SPZ_H_PDB666 1    /* src/24puzzle_with_pdb.cc:114-194 (width 5) */
+
+/* count modes */
+#define SPZ_MODE_ALL 0   /* exhaust the final iteration: src/24puzzle_expand.cc */
+#define SPZ_MODE_FIRST 1 /* stop at a goal: src/24puzzle.cc (final count partial) */
+
+/* dispatch order of frontier nodes (the reference's static LB is
+ * src/15puzzle_block_parallel.cu:490-563) */
+#define SPZ_LB_UNIFORM 0 /* frontier order */
+#define SPZ_LB_LEARNED 1 /* longest predicted subtree first (needs weights) */
+
+/* per-instance status */
+#define SPZ_ST_SOLVED 0
+#define SPZ_ST_ITER_LIMIT 1 /* max_iters iterations without a goal */
+
+typedef struct spz_ctx spz_ctx;
+
+typedef struct spz_config {
+    int32_t width;           /* 4 (15-puzzle) or 5 (24-puzzle) */
+    int32_t heuristic;       /* SPZ_H_* */
+    int32_t mode;            /* SPZ_MODE_* */
+    int32_t frontier_target; /* root split: per instance, expand the search
+                                tree breadth-first until a level holds at
+                                least this many nodes (reference: 2048,
+                                src/15puzzle_block_parallel.cu:150-187);
+                                1 = no split */
+    int32_t max_split_depth; /* cap on that split depth (0 = 24) */
+    int32_t workgroup_size;  /* 64, 128 or 256 lanes (0 = 256) */
+    int32_t lanes;           /* DFS lanes in flight (0 = fill the device) */
+    int32_t round_budget;    /* DFS steps per lane per round (0 = auto) */
+    int32_t lb_policy;       /* SPZ_LB_* */
+    int32_t device;          /* HIP device ordinal */
+    int64_t queue_items;     /* work-queue capacity in items (0 = auto) */
+} spz_config;
+
+typedef struct spz_result {
+    int32_t opt_len; /* optimal length, -1 if not found within max_iters */
+    int32_t h0;      /* heuristic value of the start board */
+    int32_t n_iters; /* iterations run (thresholds h0, h0+2, ...) */
+    int32_t status;  /* SPZ_ST_* */
+    uint64_t total_nodes; /* sum of the per-iteration counts */
+} spz_result;
+
+/* Fill *cfg with defaults for a width/heuristic (no device work). */
+void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic);
+
+/* Create a context on cfg->device.  Replaces the per-process globals and
+ * cudaMalloc of src/15puzzle_block_parallel.cu:424-452. */
+int spz_create(spz_ctx **ctx, const spz_config *cfg);
+
+/* Build the 6-6-6-6 pattern databases on the GPU by backward 0-1 BFS from
+ * the goal (blank at cell 0).  Replaces loading Korf's pat24.*.tab files
+ * (src/24puzzle_with_pdb.cc:77-112; src/24puzzle_block_parallel_with_pdb.cu:
+ * 164-230,943-944).  cache_path: if non-NULL and the file exists, the two
+ * canonical tables are read from it instead; if it does not exist they are
+ * written there after the build. */
+int spz_build_pdb(spz_ctx *ctx, const char *cache_path);
+
+/* Write the two canonical tables in the reference's .tab layout (one byte
+ * per placement, nested-loop order, src/24puzzle_with_pdb.cc:87-112). */
+int spz_export_pdb_tab(spz_ctx *ctx, const char *path_1256712,
+                       const char *path_34891314);
+
+/* Copy a canonical table (pattern 0: tiles 1,2,5,6,7,12; pattern 1: tiles
+ * 3,4,8,9,13,14) to host memory (127,512,000 bytes). */
+int spz_get_pdb(spz_ctx *ctx, int32_t pattern, uint8_t *out);
+
+/* Solve n boards.  Replaces ida_star() of src/24puzzle_expand.cc:146-164
+ * (all-solutions) / src/24puzzle.cc:146-162 (first-solution) and the GPU
+ * driver src/15puzzle_block_parallel.cu:429-565, for a whole batch in one
+ * device-resident search.  out[i] receives instance i's result;
+ * iter_counts (may be NULL) receives n*max_iters u64 per-iteration counts,
+ * row i = thresholds h0_i, h0_i+2, ... (unused entries zero). */
+int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n,
+                    int32_t max_iters, spz_result *out, uint64_t *iter_counts);
+
+/* Heuristic value of boards (Manhattan, or PDB after spz_build_pdb),
+ * computed on the device; h must hold n int32. */
+int spz_eval_h(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h);
+
+/* Change the dispatch policy / learned-LB weights between solves.
+ * weights: bf16 MLP parameters in the layout of spz_mlp_param_count(). */
+int spz_set_lb(spz_ctx *ctx, int32_t lb_policy);
+
+/* Statistics of the last solve: rounds run, kernel milliseconds, items
+ * donated, lanes; stats must hold 8 doubles. */
+int spz_last_stats(spz_ctx *ctx, double *stats);
+
+const char *spz_last_error(spz_ctx *ctx);
+void spz_destroy(spz_ctx *ctx);
+
+/* Diagnostics: the eight PDB lookups as (tiles in slot order, cell map),
+ * derived from the cell symmetries; the reference writes them out as
+ * hash0..hashref3 (src/24puzzle_with_pdb.cc:114-189).  tiles48 = 8x6,
+ * maps200 = 8x25 (either may be NULL). */
+void spz_pdb_geometry(uint8_t *tiles48, uint8_t *maps200);
+
+/* Seeded instance generators (splitmix64-seeded xoshiro256**):
+ * a non-backtracking walk of `steps` blank moves from the goal, or a
+ * uniformly random solvable permutation.  The reference's generator
+ * (src/make_24puzzle_problems.cc:54-77) backtracks; these do not. */
+void spz_gen_walk(int32_t width, int32_t steps, uint64_t seed, uint8_t *out);
+void spz_gen_random(int32_t width, uint64_t seed, uint8_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPZ_H */

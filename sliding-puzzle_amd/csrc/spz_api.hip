@@ -1,0 +1,682 @@
+// spz_api.hip -- host implementation of the libspz C ABI (include/spz.h).
+#include "../../include/spz.h"
+#include "spz_board.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+#include <vector>
+
+namespace spz {
+hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo, int wg, hipStream_t s);
+size_t round_lds_bytes(int width, int wg);
+const void *round_kernel_ptr(int width, int heur);
+hipError_t launch_restart(const KParams &p, hipStream_t s);
+hipError_t launch_control(const KParams &p, hipStream_t s);
+hipError_t launch_split_level(int width, int heur, const KParams &p, const PdbGeo *geo, const Item *lvl_in,
+                              const unsigned *n_in, Item *lvl_out, unsigned *n_out, unsigned *nfront,
+                              unsigned *topgen, int *top_goal, int *frozen, int depth, int max_depth,
+                              unsigned target, hipStream_t s);
+hipError_t launch_eval_h(int width, int heur, const KParams &p, const PdbGeo *geo, Item *items, int n,
+                         hipStream_t s);
+hipError_t launch_scatter_roots(const Item *roots, Item *lvl, int capl, int n, hipStream_t s);
+hipError_t pdb_bfs_build(int pat, uint8_t *compact, uint32_t *work, unsigned long long *d_cnt, hipStream_t s,
+                         int *levels_out);
+hipError_t pdb_spread(const uint8_t *compact, uint8_t *direct, hipStream_t s);
+constexpr uint64_t kPlacementsHost = 127512000ull;
+} // namespace spz
+
+using namespace spz;
+
+namespace {
+
+// device buffer that grows on demand
+struct DBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= n) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+} // namespace
+
+struct spz_ctx {
+    spz_config cfg;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    int n_lanes = 0;
+    int wg = 256;
+    std::string err;
+    double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double timeout_s = 0;
+
+    PdbGeo geo_h;
+    DBuf geo_d;
+    DBuf pdb_compact[2]; // file-order tables
+    DBuf pdb_direct[2];  // 25^6 addressing used by the search
+    bool have_pdb = false;
+
+    // lane contexts
+    DBuf c_st, c_rows, c_root, c_meta, c_rmeta, c_inst, c_cnt, c_pdb, c_stack;
+    // queue and control
+    DBuf q, q_ctl, ctl;
+    size_t qcap = 0;
+    // per instance
+    DBuf roots, L, it, cnt, pend, found, done, opt, iter_counts, restart_list;
+    DBuf frontier, nfront, topgen, top_goal, frozen, lvl[2], nlvl[2];
+};
+
+static int fail(spz_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                       \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess) return fail(ctx, SPZ_E_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                             \
+    } while (0)
+
+static bool is_solvable(int w, const uint8_t *b) {
+    int n2 = w * w;
+    bool seen[25] = {false};
+    int inv = 0, blank = -1;
+    for (int i = 0; i < n2; ++i) {
+        if (b[i] >= n2 || seen[b[i]]) return false;
+        seen[b[i]] = true;
+        if (b[i] == 0) blank = i;
+        for (int j = i + 1; j < n2; ++j)
+            if (b[i] > b[j]) ++inv;
+    }
+    // every move flips both the permutation parity and the blank's row+col parity
+    return (inv & 1) == ((blank / w + blank % w) & 1);
+}
+
+static void pack_root(int w, const uint8_t *b, int inst, Item *out) {
+    int blank = 0;
+    for (int c = 0; c < w * w; ++c)
+        if (b[c] == 0) blank = c;
+    if (w == 4) {
+        Board<4> bd;
+        board_from_bytes<4>(b, bd);
+        bd.pack(out->b);
+    } else {
+        Board<5> bd;
+        board_from_bytes<5>(b, bd);
+        bd.pack(out->b);
+    }
+    out->meta = meta_pack(blank, 0, 0, DIR_NONE);
+    out->inst = (uint32_t)inst;
+    out->x0 = out->x1 = 0;
+}
+
+extern "C" {
+
+void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic) {
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->width = width;
+    cfg->heuristic = heuristic;
+    cfg->mode = SPZ_MODE_ALL;
+    cfg->frontier_target = 2048;
+    cfg->max_split_depth = 24;
+    cfg->workgroup_size = 256;
+    cfg->lanes = 0;
+    cfg->round_budget = 0;
+    cfg->lb_policy = SPZ_LB_UNIFORM;
+    cfg->device = 0;
+    cfg->queue_items = 0;
+}
+
+int spz_create(spz_ctx **out, const spz_config *cfg) {
+    spz_ctx *ctx = nullptr;
+    if (!out || !cfg) return SPZ_E_ARG;
+    *out = nullptr;
+    if (cfg->width != 4 && cfg->width != 5) return SPZ_E_ARG;
+    if (cfg->heuristic == SPZ_H_PDB666 && cfg->width != 5) return SPZ_E_ARG;
+    if (cfg->heuristic != SPZ_H_MANHATTAN && cfg->heuristic != SPZ_H_PDB666) return SPZ_E_ARG;
+    int wg = cfg->workgroup_size ? cfg->workgroup_size : 256;
+    if (wg != 64 && wg != 128 && wg != 256) return SPZ_E_ARG;
+    ctx = new spz_ctx();
+    ctx->cfg = *cfg;
+    ctx->wg = wg;
+    {
+        const char *t = getenv("SPZ_TIMEOUT_S");
+        ctx->timeout_s = t ? atof(t) : 900.0;
+    }
+    if (ctx->cfg.frontier_target < 1) ctx->cfg.frontier_target = 1;
+    if (ctx->cfg.max_split_depth <= 0) ctx->cfg.max_split_depth = 24;
+    int rc = SPZ_OK;
+    do {
+        hipError_t e = hipSetDevice(cfg->device);
+        if (e != hipSuccess) {
+            rc = fail(ctx, SPZ_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+            break;
+        }
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, cfg->device);
+        if (e != hipSuccess) {
+            rc = fail(ctx, SPZ_E_HIP, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+            break;
+        }
+        ctx->n_cu = prop.multiProcessorCount;
+        e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            rc = fail(ctx, SPZ_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+            break;
+        }
+        int lanes = cfg->lanes;
+        if (lanes <= 0) {
+            int blocks = 0;
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &blocks, round_kernel_ptr(cfg->width, cfg->heuristic), wg, round_lds_bytes(cfg->width, wg));
+            if (e != hipSuccess || blocks <= 0) blocks = 2;
+            lanes = ctx->n_cu * blocks * wg;
+        }
+        lanes = (lanes + wg - 1) / wg * wg;
+        ctx->n_lanes = lanes;
+        pdb_geo_build(&ctx->geo_h);
+        if ((e = ctx->geo_d.ensure(sizeof(PdbGeo))) != hipSuccess ||
+            (e = hipMemcpy(ctx->geo_d.p, &ctx->geo_h, sizeof(PdbGeo), hipMemcpyHostToDevice)) != hipSuccess) {
+            rc = fail(ctx, SPZ_E_HIP, "geometry upload: %s", hipGetErrorString(e));
+            break;
+        }
+        size_t L = (size_t)lanes;
+        int maxd = cfg->width == 4 ? Geo<4>::MAXD : Geo<5>::MAXD;
+        if ((e = ctx->c_st.ensure(L * 4)) != hipSuccess || (e = ctx->c_rows.ensure(L * 4 * 5)) != hipSuccess ||
+            (e = ctx->c_root.ensure(L * 4 * 4)) != hipSuccess || (e = ctx->c_meta.ensure(L * 4)) != hipSuccess ||
+            (e = ctx->c_rmeta.ensure(L * 4)) != hipSuccess || (e = ctx->c_inst.ensure(L * 4)) != hipSuccess ||
+            (e = ctx->c_cnt.ensure(L * 4)) != hipSuccess || (e = ctx->c_pdb.ensure(L * 4 * 10)) != hipSuccess ||
+            (e = ctx->c_stack.ensure(L * (size_t)maxd)) != hipSuccess || (e = ctx->q_ctl.ensure(64)) != hipSuccess ||
+            (e = ctx->ctl.ensure(64 * 4)) != hipSuccess) {
+            rc = fail(ctx, SPZ_E_HIP, "context allocation: %s", hipGetErrorString(e));
+            break;
+        }
+    } while (0);
+    if (rc != SPZ_OK) {
+        // keep the context so the caller can read the error, but report failure
+        *out = ctx;
+        return rc;
+    }
+    *out = ctx;
+    return SPZ_OK;
+}
+
+const char *spz_last_error(spz_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void spz_destroy(spz_ctx *ctx) {
+    if (!ctx) return;
+    DBuf *all[] = {&ctx->geo_d,       &ctx->pdb_compact[0], &ctx->pdb_compact[1], &ctx->pdb_direct[0],
+                   &ctx->pdb_direct[1], &ctx->c_st,         &ctx->c_rows,         &ctx->c_root,
+                   &ctx->c_meta,      &ctx->c_rmeta,        &ctx->c_inst,         &ctx->c_cnt,
+                   &ctx->c_pdb,       &ctx->c_stack,        &ctx->q,              &ctx->q_ctl,
+                   &ctx->ctl,         &ctx->roots,          &ctx->L,              &ctx->it,
+                   &ctx->cnt,         &ctx->pend,           &ctx->found,          &ctx->done,
+                   &ctx->opt,         &ctx->iter_counts,    &ctx->restart_list,   &ctx->frontier,
+                   &ctx->nfront,      &ctx->topgen,         &ctx->top_goal,       &ctx->frozen,
+                   &ctx->lvl[0],      &ctx->lvl[1],         &ctx->nlvl[0],        &ctx->nlvl[1]};
+    for (DBuf *b : all) b->release();
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int spz_last_stats(spz_ctx *ctx, double *stats) {
+    if (!ctx || !stats) return SPZ_E_ARG;
+    memcpy(stats, ctx->stats, sizeof ctx->stats);
+    return SPZ_OK;
+}
+
+int spz_set_lb(spz_ctx *ctx, int32_t lb_policy) {
+    if (!ctx) return SPZ_E_ARG;
+    if (lb_policy != SPZ_LB_UNIFORM && lb_policy != SPZ_LB_LEARNED)
+        return fail(ctx, SPZ_E_ARG, "unknown lb policy %d", lb_policy);
+    ctx->cfg.lb_policy = lb_policy;
+    return SPZ_OK;
+}
+
+// ---------------------------------------------------------------- PDB ---
+static int upload_direct(spz_ctx *ctx) {
+    for (int t = 0; t < 2; ++t) {
+        HIPCHK(ctx->pdb_direct[t].ensure(kPdbDirect));
+        HIPCHK(pdb_spread(ctx->pdb_compact[t].as<uint8_t>(), ctx->pdb_direct[t].as<uint8_t>(), ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->have_pdb = true;
+    return SPZ_OK;
+}
+
+int spz_build_pdb(spz_ctx *ctx, const char *cache_path) {
+    if (!ctx) return SPZ_E_ARG;
+    if (ctx->cfg.width != 5) return fail(ctx, SPZ_E_ARG, "PDBs exist for the 24-puzzle only");
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    for (int t = 0; t < 2; ++t) HIPCHK(ctx->pdb_compact[t].ensure(kPlacementsHost));
+    if (cache_path) {
+        FILE *f = fopen(cache_path, "rb");
+        if (f) {
+            std::vector<uint8_t> buf(kPlacementsHost);
+            for (int t = 0; t < 2; ++t) {
+                if (fread(buf.data(), 1, buf.size(), f) != buf.size()) {
+                    fclose(f);
+                    return fail(ctx, SPZ_E_IO, "short PDB cache file %s", cache_path);
+                }
+                HIPCHK(hipMemcpy(ctx->pdb_compact[t].p, buf.data(), buf.size(), hipMemcpyHostToDevice));
+            }
+            fclose(f);
+            return upload_direct(ctx);
+        }
+    }
+    DBuf work, cntb;
+    hipError_t e = work.ensure(3ull * kPlacementsHost * 4);
+    if (e == hipSuccess) e = cntb.ensure(8);
+    if (e != hipSuccess) {
+        work.release();
+        cntb.release();
+        return fail(ctx, SPZ_E_HIP, "PDB scratch allocation: %s", hipGetErrorString(e));
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    for (int t = 0; t < 2; ++t) {
+        int levels = 0;
+        e = pdb_bfs_build(t, ctx->pdb_compact[t].as<uint8_t>(), work.as<uint32_t>(),
+                          cntb.as<unsigned long long>(), ctx->stream, &levels);
+        if (e != hipSuccess) break;
+        ctx->stats[4 + t] = levels;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    ctx->stats[6] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    work.release();
+    cntb.release();
+    if (e != hipSuccess) return fail(ctx, SPZ_E_HIP, "PDB BFS: %s", hipGetErrorString(e));
+    int rc = upload_direct(ctx);
+    if (rc != SPZ_OK) return rc;
+    if (cache_path) {
+        FILE *f = fopen(cache_path, "wb");
+        if (!f) return fail(ctx, SPZ_E_IO, "cannot write PDB cache %s", cache_path);
+        std::vector<uint8_t> buf(kPlacementsHost);
+        for (int t = 0; t < 2; ++t) {
+            HIPCHK(hipMemcpy(buf.data(), ctx->pdb_compact[t].p, buf.size(), hipMemcpyDeviceToHost));
+            if (fwrite(buf.data(), 1, buf.size(), f) != buf.size()) {
+                fclose(f);
+                return fail(ctx, SPZ_E_IO, "short write to %s", cache_path);
+            }
+        }
+        fclose(f);
+    }
+    return SPZ_OK;
+}
+
+int spz_get_pdb(spz_ctx *ctx, int32_t pattern, uint8_t *out) {
+    if (!ctx || !out || pattern < 0 || pattern > 1) return SPZ_E_ARG;
+    if (!ctx->have_pdb) return fail(ctx, SPZ_E_NOPDB, "no PDB built");
+    HIPCHK(hipMemcpy(out, ctx->pdb_compact[pattern].p, kPlacementsHost, hipMemcpyDeviceToHost));
+    return SPZ_OK;
+}
+
+int spz_export_pdb_tab(spz_ctx *ctx, const char *p0, const char *p1) {
+    if (!ctx || !p0 || !p1) return SPZ_E_ARG;
+    if (!ctx->have_pdb) return fail(ctx, SPZ_E_NOPDB, "no PDB built");
+    std::vector<uint8_t> buf(kPlacementsHost);
+    const char *paths[2] = {p0, p1};
+    for (int t = 0; t < 2; ++t) {
+        HIPCHK(hipMemcpy(buf.data(), ctx->pdb_compact[t].p, buf.size(), hipMemcpyDeviceToHost));
+        FILE *f = fopen(paths[t], "wb");
+        if (!f) return fail(ctx, SPZ_E_IO, "cannot write %s", paths[t]);
+        size_t w = fwrite(buf.data(), 1, buf.size(), f);
+        fclose(f);
+        if (w != buf.size()) return fail(ctx, SPZ_E_IO, "short write to %s", paths[t]);
+    }
+    return SPZ_OK;
+}
+
+// -------------------------------------------------------------- solve ---
+static KParams base_params(spz_ctx *ctx, int n, int max_iters, int capl) {
+    KParams p;
+    memset(&p, 0, sizeof p);
+    p.q = ctx->q.as<Item>();
+    p.qmask = ctx->qcap - 1;
+    p.q_ctl = ctx->q_ctl.as<unsigned long long>();
+    p.n_inst = n;
+    p.max_iters = max_iters;
+    p.mode = ctx->cfg.mode;
+    p.roots = ctx->roots.as<Item>();
+    p.L = ctx->L.as<int>();
+    p.it = ctx->it.as<int>();
+    p.cnt = ctx->cnt.as<unsigned long long>();
+    p.pend = ctx->pend.as<unsigned>();
+    p.found = ctx->found.as<int>();
+    p.done = ctx->done.as<int>();
+    p.opt = ctx->opt.as<int>();
+    p.iter_counts = ctx->iter_counts.as<unsigned long long>();
+    p.restart_list = ctx->restart_list.as<int>();
+    p.ctl = ctx->ctl.as<unsigned>();
+    p.frontier = ctx->frontier.as<Item>();
+    p.nfront = ctx->nfront.as<unsigned>();
+    p.topgen = ctx->topgen.as<unsigned>();
+    p.top_goal = ctx->top_goal.as<int>();
+    p.capl = capl;
+    p.n_lanes = ctx->n_lanes;
+    p.c_st = ctx->c_st.as<uint32_t>();
+    p.c_rows = ctx->c_rows.as<uint32_t>();
+    p.c_root = ctx->c_root.as<uint32_t>();
+    p.c_meta = ctx->c_meta.as<uint32_t>();
+    p.c_rmeta = ctx->c_rmeta.as<uint32_t>();
+    p.c_inst = ctx->c_inst.as<uint32_t>();
+    p.c_cnt = ctx->c_cnt.as<uint32_t>();
+    p.c_pdb = ctx->c_pdb.as<uint32_t>();
+    p.c_stack = ctx->c_stack.as<uint8_t>();
+    p.pdb0 = ctx->pdb_direct[0].as<uint8_t>();
+    p.pdb1 = ctx->pdb_direct[1].as<uint8_t>();
+    return p;
+}
+
+static size_t next_pow2(size_t x) {
+    size_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+int spz_eval_h(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h) {
+    if (!ctx || !boards || !h || n < 0) return SPZ_E_ARG;
+    if (n == 0) return SPZ_OK;
+    const int w = ctx->cfg.width, heur = ctx->cfg.heuristic;
+    if (heur == SPZ_H_PDB666 && !ctx->have_pdb) return fail(ctx, SPZ_E_NOPDB, "call spz_build_pdb first");
+    for (int i = 0; i < n; ++i)
+        if (!is_solvable(w, boards + (size_t)i * w * w))
+            return fail(ctx, SPZ_E_ARG, "board %d is not a solvable permutation", i);
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    std::vector<Item> items((size_t)n);
+    for (int i = 0; i < n; ++i) pack_root(w, boards + (size_t)i * w * w, i, &items[i]);
+    DBuf d;
+    HIPCHK(d.ensure(sizeof(Item) * (size_t)n));
+    KParams p = base_params(ctx, 0, 1, 1);
+    hipError_t e = hipMemcpyAsync(d.p, items.data(), sizeof(Item) * n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_eval_h(w, heur, p, ctx->geo_d.as<PdbGeo>(), d.as<Item>(), n, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(items.data(), d.p, sizeof(Item) * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    d.release();
+    if (e != hipSuccess) return fail(ctx, SPZ_E_HIP, "eval_h: %s", hipGetErrorString(e));
+    for (int i = 0; i < n; ++i) h[i] = meta_h(items[i].meta);
+    return SPZ_OK;
+}
+
+int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_iters, spz_result *out,
+                    uint64_t *iter_counts) {
+    if (!ctx || !boards || !out || n < 0) return SPZ_E_ARG;
+    if (max_iters < 1 || max_iters > kMaxIters)
+        return fail(ctx, SPZ_E_ARG, "max_iters must be in 1..%d", kMaxIters);
+    if (n == 0) return SPZ_OK;
+    if (n > (1 << 24)) return fail(ctx, SPZ_E_ARG, "batch too large");
+    const int w = ctx->cfg.width, heur = ctx->cfg.heuristic;
+    if (heur == SPZ_H_PDB666 && !ctx->have_pdb) return fail(ctx, SPZ_E_NOPDB, "call spz_build_pdb first");
+    for (int i = 0; i < n; ++i)
+        if (!is_solvable(w, boards + (size_t)i * w * w))
+            return fail(ctx, SPZ_E_ARG, "board %d is not a solvable permutation", i);
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    hipStream_t s = ctx->stream;
+    auto t_start = std::chrono::steady_clock::now();
+
+    const unsigned target = (unsigned)ctx->cfg.frontier_target;
+    const int capl = (int)std::max<unsigned>(4u, 4u * target);
+    const size_t N = (size_t)n;
+    const size_t front_items = N * (size_t)capl;
+    size_t qcap = ctx->cfg.queue_items > 0 ? (size_t)ctx->cfg.queue_items
+                                           : std::max<size_t>(1u << 22, 4 * front_items + 8 * (size_t)ctx->n_lanes);
+    qcap = next_pow2(qcap);
+    ctx->qcap = qcap;
+
+    HIPCHK(ctx->q.ensure(qcap * sizeof(Item)));
+    HIPCHK(ctx->roots.ensure(N * sizeof(Item)));
+    HIPCHK(ctx->L.ensure(N * 4));
+    HIPCHK(ctx->it.ensure(N * 4));
+    HIPCHK(ctx->cnt.ensure(N * 8));
+    HIPCHK(ctx->pend.ensure(N * 4));
+    HIPCHK(ctx->found.ensure(N * 4));
+    HIPCHK(ctx->done.ensure(N * 4));
+    HIPCHK(ctx->opt.ensure(N * 4));
+    HIPCHK(ctx->iter_counts.ensure(N * (size_t)max_iters * 8));
+    HIPCHK(ctx->restart_list.ensure(N * 4));
+    HIPCHK(ctx->frontier.ensure(front_items * sizeof(Item)));
+    HIPCHK(ctx->nfront.ensure(N * 4));
+    HIPCHK(ctx->topgen.ensure(N * kFBins * 4));
+    HIPCHK(ctx->top_goal.ensure(N * 4));
+    HIPCHK(ctx->frozen.ensure(N * 4));
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(ctx->lvl[k].ensure(front_items * sizeof(Item)));
+        HIPCHK(ctx->nlvl[k].ensure(N * 4));
+    }
+
+    KParams p = base_params(ctx, n, max_iters, capl);
+
+    // roots and their heuristic
+    std::vector<Item> roots(N);
+    for (int i = 0; i < n; ++i) pack_root(w, boards + (size_t)i * w * w, i, &roots[i]);
+    HIPCHK(hipMemcpyAsync(ctx->roots.p, roots.data(), N * sizeof(Item), hipMemcpyHostToDevice, s));
+    HIPCHK(launch_eval_h(w, heur, p, ctx->geo_d.as<PdbGeo>(), ctx->roots.as<Item>(), n, s));
+    HIPCHK(hipMemcpyAsync(roots.data(), ctx->roots.p, N * sizeof(Item), hipMemcpyDeviceToHost, s));
+
+    // root split: level 0 holds the root of each instance at slot 0
+    HIPCHK(hipMemsetAsync(ctx->topgen.p, 0, N * kFBins * 4, s));
+    HIPCHK(hipMemsetAsync(ctx->frozen.p, 0, N * 4, s));
+    HIPCHK(hipMemsetAsync(ctx->nfront.p, 0, N * 4, s));
+    HIPCHK(hipStreamSynchronize(s));
+    {
+        std::vector<int> tg(N, kNoGoal);
+        HIPCHK(hipMemcpyAsync(ctx->top_goal.p, tg.data(), N * 4, hipMemcpyHostToDevice, s));
+        std::vector<unsigned> ones(N, 1u);
+        HIPCHK(hipMemcpyAsync(ctx->nlvl[0].p, ones.data(), N * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(launch_scatter_roots(ctx->roots.as<Item>(), ctx->lvl[0].as<Item>(), capl, n, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    const int max_depth = ctx->cfg.max_split_depth;
+    int cur = 0;
+    for (int depth = 0; depth <= max_depth; ++depth) {
+        HIPCHK(hipMemsetAsync(ctx->nlvl[cur ^ 1].p, 0, N * 4, s));
+        HIPCHK(launch_split_level(w, heur, p, ctx->geo_d.as<PdbGeo>(), ctx->lvl[cur].as<Item>(),
+                                  ctx->nlvl[cur].as<unsigned>(), ctx->lvl[cur ^ 1].as<Item>(),
+                                  ctx->nlvl[cur ^ 1].as<unsigned>(), ctx->nfront.as<unsigned>(),
+                                  ctx->topgen.as<unsigned>(), ctx->top_goal.as<int>(), ctx->frozen.as<int>(),
+                                  depth, max_depth, target, s));
+        cur ^= 1;
+    }
+
+    // per-instance iteration state
+    {
+        std::vector<int> Lh(N), zero(N, 0), nog(N, kNoGoal), neg(N, -1), list(N);
+        for (int i = 0; i < n; ++i) {
+            Lh[i] = meta_h(roots[i].meta);
+            list[i] = i;
+        }
+        HIPCHK(hipMemcpyAsync(ctx->L.p, Lh.data(), N * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(ctx->it.p, zero.data(), N * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(ctx->cnt.p, 0, N * 8, s));
+        HIPCHK(hipMemsetAsync(ctx->pend.p, 0, N * 4, s));
+        HIPCHK(hipMemcpyAsync(ctx->found.p, nog.data(), N * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(ctx->done.p, 0, N * 4, s));
+        HIPCHK(hipMemcpyAsync(ctx->opt.p, neg.data(), N * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(ctx->iter_counts.p, 0, N * (size_t)max_iters * 8, s));
+        HIPCHK(hipMemcpyAsync(ctx->restart_list.p, list.data(), N * 4, hipMemcpyHostToDevice, s));
+        unsigned ctl[64];
+        memset(ctl, 0, sizeof ctl);
+        ctl[0] = (unsigned)n; // restart list length
+        ctl[1] = (unsigned)n; // active instances
+        int big = ctx->cfg.round_budget > 0 ? ctx->cfg.round_budget : 1024;
+        ctl[9] = (unsigned)big;
+        ctl[10] = (unsigned)std::min(big, 32);
+        ctl[5] = ctl[10];
+        HIPCHK(hipMemcpyAsync(ctx->ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(ctx->q_ctl.p, 0, 64, s));
+        HIPCHK(hipMemsetAsync(ctx->c_st.p, 0, (size_t)ctx->n_lanes * 4, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+
+    HIPCHK(launch_restart(p, s));
+    HIPCHK(launch_control(p, s));
+
+    // rounds until every instance is done; poll every few rounds.  Each
+    // round is a bounded kernel; the host stops on a wall-clock limit or if
+    // work vanishes while instances are still active (a bookkeeping bug).
+    const int per_poll = 4;
+    const double limit_s = ctx->timeout_s;
+    int stalled = 0;
+    unsigned *h_ctl = nullptr;
+    HIPCHK(hipHostMalloc((void **)&h_ctl, 64 * 4, hipHostMallocDefault));
+    int rc = SPZ_OK;
+    long rounds = 0;
+    for (;;) {
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < per_poll && e == hipSuccess; ++k) {
+            e = launch_round(w, heur, p, ctx->geo_d.as<PdbGeo>(), ctx->wg, s);
+            if (e == hipSuccess) e = launch_restart(p, s);
+            if (e == hipSuccess) e = launch_control(p, s);
+        }
+        rounds += per_poll;
+        if (e == hipSuccess) e = hipMemcpyAsync(h_ctl, ctx->ctl.p, 64 * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            rc = fail(ctx, SPZ_E_HIP, "search rounds: %s", hipGetErrorString(e));
+            break;
+        }
+        if (h_ctl[1] == 0) break;
+        unsigned long long qc[3];
+        e = hipMemcpy(qc, ctx->q_ctl.p, sizeof qc, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            rc = fail(ctx, SPZ_E_HIP, "search rounds: %s", hipGetErrorString(e));
+            break;
+        }
+        stalled = (qc[2] == qc[0] && h_ctl[2] == 0 && h_ctl[0] == 0) ? stalled + 1 : 0;
+        if (stalled > 8) {
+            rc = fail(ctx, SPZ_E_STALL, "search stalled with %u active instances", h_ctl[1]);
+            break;
+        }
+        double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        if (limit_s > 0 && el > limit_s) {
+            rc = fail(ctx, SPZ_E_TIMEOUT, "search exceeded %.0f s", limit_s);
+            break;
+        }
+    }
+    if (rc != SPZ_OK) {
+        hipHostFree(h_ctl);
+        return rc;
+    }
+    ctx->stats[0] = (double)h_ctl[6];
+    ctx->stats[2] = (double)h_ctl[7];
+    ctx->stats[3] = (double)ctx->n_lanes;
+    ctx->stats[7] = (double)h_ctl[8];
+    hipHostFree(h_ctl);
+
+    std::vector<int> opt(N), its(N);
+    std::vector<unsigned long long> ic(N * (size_t)max_iters);
+    HIPCHK(hipMemcpyAsync(opt.data(), ctx->opt.p, N * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(its.data(), ctx->it.p, N * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(ic.data(), ctx->iter_counts.p, ic.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    ctx->stats[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    for (int i = 0; i < n; ++i) {
+        spz_result &r = out[i];
+        r.opt_len = opt[i];
+        r.h0 = meta_h(roots[i].meta);
+        r.n_iters = its[i] + 1;
+        r.status = opt[i] >= 0 ? SPZ_ST_SOLVED : SPZ_ST_ITER_LIMIT;
+        uint64_t tot = 0;
+        for (int k = 0; k < r.n_iters; ++k) tot += ic[(size_t)i * max_iters + k];
+        r.total_nodes = tot;
+        if (iter_counts)
+            for (int k = 0; k < max_iters; ++k) iter_counts[(size_t)i * max_iters + k] = ic[(size_t)i * max_iters + k];
+    }
+    return SPZ_OK;
+}
+
+void spz_pdb_geometry(uint8_t *tiles48, uint8_t *maps200) {
+    PdbGeo g;
+    pdb_geo_build(&g);
+    if (tiles48) memcpy(tiles48, g.tiles, 48);
+    if (maps200) memcpy(maps200, g.map, 200);
+}
+
+// ----------------------------------------------------------- generators ---
+static uint64_t splitmix64(uint64_t *x) {
+    uint64_t z = (*x += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+struct Xoshiro {
+    uint64_t s[4];
+    explicit Xoshiro(uint64_t seed) {
+        uint64_t x = seed;
+        for (auto &v : s) v = splitmix64(&x);
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next() {
+        uint64_t r = rotl(s[1] * 5, 7) * 9, t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    int below(int k) { return (int)(((next() >> 32) * (uint64_t)k) >> 32); }
+};
+
+void spz_gen_walk(int32_t width, int32_t steps, uint64_t seed, uint8_t *out) {
+    int n2 = width * width, blank = 0, prev = -1;
+    Xoshiro r(seed);
+    for (int i = 0; i < n2; ++i) out[i] = (uint8_t)i;
+    for (int s = 0; s < steps; ++s) {
+        int cand[4], nc = 0;
+        int row = blank / width, col = blank % width;
+        for (int d = 0; d < 4; ++d) {
+            if (prev >= 0 && d == 3 - prev) continue;
+            bool ok = (d == DIR_UP && row > 0) || (d == DIR_RIGHT && col < width - 1) ||
+                      (d == DIR_LEFT && col > 0) || (d == DIR_DOWN && row < width - 1);
+            if (ok) cand[nc++] = d;
+        }
+        int d = cand[r.below(nc)];
+        int nb = blank + (d == DIR_UP ? -width : d == DIR_DOWN ? width : d == DIR_LEFT ? -1 : 1);
+        out[blank] = out[nb];
+        out[nb] = 0;
+        blank = nb;
+        prev = d;
+    }
+}
+
+void spz_gen_random(int32_t width, uint64_t seed, uint8_t *out) {
+    int n2 = width * width;
+    Xoshiro r(seed);
+    for (;;) {
+        for (int i = 0; i < n2; ++i) out[i] = (uint8_t)i;
+        for (int i = n2 - 1; i > 0; --i) {
+            int j = r.below(i + 1);
+            std::swap(out[i], out[j]);
+        }
+        if (is_solvable(width, out)) return;
+    }
+}
+
+} // extern "C"

@@ -1,0 +1,260 @@
+// spz_board.h -- per-lane puzzle state for the DFS engine (device code).
+//
+// A board lives in scalar registers only (no arrays, so nothing is placed in
+// scratch): the 15-puzzle as one u64 of 4-bit cells, the 24-puzzle as two
+// u64 of twelve 5-bit cells (cells 0-11, 12-23) plus one word for cell 24.
+// The heuristic state rides along: Manhattan h updated by +-1 per move
+// (src/24puzzle.cc:129-131), or the eight PDB lookup indices/values updated
+// incrementally, re-reading only the two lookups that contain the moved tile
+// (src_horie/idas_pdb.cu:220-255).
+#pragma once
+#include "spz_internal.h"
+
+namespace spz {
+
+template <int W> struct Geo;
+template <> struct Geo<4> {
+    static constexpr int N2 = 16;
+    static constexpr int MAXD = 96; // LDS stack levels per lane
+};
+template <> struct Geo<5> {
+    static constexpr int N2 = 25;
+    static constexpr int MAXD = 160;
+};
+
+// goal row / column of tile t (tile t belongs at cell t)
+template <int W> __host__ __device__ __forceinline__ int trow(int t) {
+    return W == 4 ? (t >> 2) : ((t * 13) >> 6); // exact for t < 25
+}
+template <int W> __host__ __device__ __forceinline__ int tcol(int t) { return t - W * trow<W>(t); }
+
+template <int W> struct Board;
+
+template <> struct Board<4> {
+    uint64_t b; // cell c at bits 4c
+    __host__ __device__ __forceinline__ int tile(int c) const { return (int)((b >> (4 * c)) & 15u); }
+    // tile t slides from cell `from` into the blank at `to`
+    __host__ __device__ __forceinline__ void slide(int t, int from, int to) {
+        b += ((uint64_t)t << (4 * to)) - ((uint64_t)t << (4 * from));
+    }
+    __host__ __device__ __forceinline__ void pack(uint32_t (&o)[4]) const {
+        o[0] = (uint32_t)b;
+        o[1] = (uint32_t)(b >> 32);
+        o[2] = 0;
+        o[3] = 0;
+    }
+    __host__ __device__ __forceinline__ void unpack(const uint32_t (&o)[4]) {
+        b = (uint64_t)o[0] | ((uint64_t)o[1] << 32);
+    }
+};
+
+template <> struct Board<5> {
+    uint64_t lo, hi; // cells 0-11 at bits 5c, cells 12-23 at bits 5(c-12)
+    uint32_t top;    // cell 24
+    __host__ __device__ __forceinline__ int tile(int c) const {
+        uint32_t a = (uint32_t)((lo >> (5 * (c < 12 ? c : 0))) & 31u);
+        uint32_t h = (uint32_t)((hi >> (5 * (c >= 12 && c < 24 ? c - 12 : 0))) & 31u);
+        return (int)(c < 12 ? a : (c < 24 ? h : top));
+    }
+    __host__ __device__ __forceinline__ void add(int c, int v) {
+        lo += (c < 12) ? ((uint64_t)v << (5 * (c < 12 ? c : 0))) : 0ull;
+        hi += (c >= 12 && c < 24) ? ((uint64_t)v << (5 * (c >= 12 && c < 24 ? c - 12 : 0))) : 0ull;
+        top += (c == 24) ? (uint32_t)v : 0u;
+    }
+    __host__ __device__ __forceinline__ void slide(int t, int from, int to) {
+        add(to, t);
+        add(from, -t);
+    }
+    // queue format: lo/hi low 60 bits, cell 24 split over their top 4 bits
+    __host__ __device__ __forceinline__ void pack(uint32_t (&o)[4]) const {
+        uint64_t l = lo | ((uint64_t)(top & 15u) << 60);
+        uint64_t h = hi | ((uint64_t)(top >> 4) << 60);
+        o[0] = (uint32_t)l;
+        o[1] = (uint32_t)(l >> 32);
+        o[2] = (uint32_t)h;
+        o[3] = (uint32_t)(h >> 32);
+    }
+    __host__ __device__ __forceinline__ void unpack(const uint32_t (&o)[4]) {
+        uint64_t l = (uint64_t)o[0] | ((uint64_t)o[1] << 32);
+        uint64_t h = (uint64_t)o[2] | ((uint64_t)o[3] << 32);
+        lo = l & ((1ull << 60) - 1);
+        hi = h & ((1ull << 60) - 1);
+        top = (uint32_t)(l >> 60) | ((uint32_t)(h >> 60) << 4);
+    }
+};
+
+// host helper: board from a row-major byte array
+template <int W> inline void board_from_bytes(const uint8_t *b, Board<W> &o);
+template <> inline void board_from_bytes<4>(const uint8_t *b, Board<4> &o) {
+    o.b = 0;
+    for (int c = 0; c < 16; ++c) o.b |= (uint64_t)b[c] << (4 * c);
+}
+template <> inline void board_from_bytes<5>(const uint8_t *b, Board<5> &o) {
+    o.lo = o.hi = 0;
+    o.top = b[24];
+    for (int c = 0; c < 12; ++c) o.lo |= (uint64_t)b[c] << (5 * c);
+    for (int c = 12; c < 24; ++c) o.hi |= (uint64_t)b[c] << (5 * (c - 12));
+}
+
+// ---------------------------------------------------------------------------
+// PDB lookup geometry.  Lookup k (0..3 regular partition, 4..7 reflected)
+// reads canonical table tab(k) (0 for k = 0,4; 1 otherwise) at the direct
+// 25^6 index sum_j map_k(pos(tile_kj)) * 25^(5-j) (src/24puzzle_with_pdb.cc:
+// 114-189 writes the same eight hash functions out by hand).
+struct PdbGeo {
+    uint8_t map[8][25];   // cell map of lookup k
+    uint8_t tiles[8][6];  // tiles of lookup k in slot order
+    uint8_t reg_k[25];    // regular lookup containing tile t (0..3)
+    uint8_t ref_k[25];    // reflected lookup containing tile t (4..7)
+    uint8_t pad[2];
+    uint32_t reg_mul[25]; // 25^(5-slot) of tile t in reg_k
+    uint32_t ref_mul[25];
+};
+
+// Build the geometry on the host from the cell symmetries (the reference's
+// rf / rot90 / rot180 tables, src/24puzzle_with_pdb.cc:33-46).
+void pdb_geo_build(PdbGeo *g);
+
+__host__ __device__ inline uint32_t pdb_tab_of(int k) { return (k == 0 || k == 4) ? 0u : 1u; }
+
+constexpr uint64_t kPdbDirect = 244140625ull; // 25^6
+
+// Per-lane heuristic state.
+template <int HEUR> struct HState;
+
+template <> struct HState<0> {
+    int h;
+};
+
+template <> struct HState<1> {
+    int h;
+    int A, B;                   // regular / reflected sums, h = max(A, B)
+    uint32_t i0, i1, i2, i3;    // direct indices, regular lookups 0..3
+    uint32_t j0, j1, j2, j3;    // direct indices, reflected lookups 4..7
+    uint32_t v0, v1;            // lookup values, bytes 0..3 = lookups 0..3 / 4..7
+};
+
+struct PdbCtx {
+    const PdbGeo *geo; // LDS copy
+    const uint8_t *t0;
+    const uint8_t *t1;
+};
+
+__device__ __forceinline__ uint32_t sel4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, int k) {
+    uint32_t lo = (k & 1) ? b : a;
+    uint32_t hi = (k & 1) ? d : c;
+    return (k & 2) ? hi : lo;
+}
+__device__ __forceinline__ void add4(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d, int k, uint32_t v) {
+    a += (k == 0) ? v : 0u;
+    b += (k == 1) ? v : 0u;
+    c += (k == 2) ? v : 0u;
+    d += (k == 3) ? v : 0u;
+}
+__device__ __forceinline__ int byte_of(uint32_t w, int k) { return (int)((w >> (8 * k)) & 0xffu); }
+__device__ __forceinline__ uint32_t set_byte(uint32_t w, int k, int v) {
+    return (w & ~(0xffu << (8 * k))) | ((uint32_t)v << (8 * k));
+}
+
+// kr in 0..3 (regular lookups: table 0 for kr == 0), kf in 0..3 (reflected
+// lookups 4..7: table 0 for kf == 0)
+__device__ __forceinline__ int pdb_read_k(const PdbCtx &P, int k03, uint32_t idx) {
+    const uint8_t *t = (k03 == 0) ? P.t0 : P.t1;
+    return (int)t[idx];
+}
+
+// Heuristic of the child reached by moving tile t from cell a to cell b,
+// without changing the state (f-tests at expansion).
+__device__ __forceinline__ int pdb_child_h(const PdbCtx &P, const HState<1> &s, int t, int a, int b) {
+    const PdbGeo &G = *P.geo;
+    int kr = G.reg_k[t], kf = G.ref_k[t] - 4;
+    uint32_t ir = sel4(s.i0, s.i1, s.i2, s.i3, kr) +
+                  (uint32_t)((int)G.map[kr][b] - (int)G.map[kr][a]) * G.reg_mul[t];
+    uint32_t iff = sel4(s.j0, s.j1, s.j2, s.j3, kf) +
+                   (uint32_t)((int)G.map[kf + 4][b] - (int)G.map[kf + 4][a]) * G.ref_mul[t];
+    int vr = pdb_read_k(P, kr, ir);
+    int vf = pdb_read_k(P, kf, iff);
+    int A = s.A - byte_of(s.v0, kr) + vr;
+    int B = s.B - byte_of(s.v1, kf) + vf;
+    return A > B ? A : B;
+}
+
+__device__ __forceinline__ void pdb_apply(const PdbCtx &P, HState<1> &s, int t, int a, int b) {
+    const PdbGeo &G = *P.geo;
+    int kr = G.reg_k[t], kf = G.ref_k[t] - 4;
+    uint32_t dr = (uint32_t)((int)G.map[kr][b] - (int)G.map[kr][a]) * G.reg_mul[t];
+    uint32_t df = (uint32_t)((int)G.map[kf + 4][b] - (int)G.map[kf + 4][a]) * G.ref_mul[t];
+    add4(s.i0, s.i1, s.i2, s.i3, kr, dr);
+    add4(s.j0, s.j1, s.j2, s.j3, kf, df);
+    int vr = pdb_read_k(P, kr, sel4(s.i0, s.i1, s.i2, s.i3, kr));
+    int vf = pdb_read_k(P, kf, sel4(s.j0, s.j1, s.j2, s.j3, kf));
+    s.A += vr - byte_of(s.v0, kr);
+    s.B += vf - byte_of(s.v1, kf);
+    s.v0 = set_byte(s.v0, kr, vr);
+    s.v1 = set_byte(s.v1, kf, vf);
+    s.h = s.A > s.B ? s.A : s.B;
+}
+
+// Full (non-incremental) initialisation of the PDB state from a board:
+// every non-blank cell adds map_k(cell) * 25^(5-slot) to the index of the
+// regular and the reflected lookup containing its tile.
+__device__ __forceinline__ void pdb_init(const PdbCtx &P, HState<1> &s, const Board<5> &bd) {
+    const PdbGeo &G = *P.geo;
+    s.i0 = s.i1 = s.i2 = s.i3 = 0;
+    s.j0 = s.j1 = s.j2 = s.j3 = 0;
+#pragma unroll
+    for (int c = 0; c < 25; ++c) {
+        int t = bd.tile(c);
+        if (t != 0) {
+            int kr = G.reg_k[t], kf = G.ref_k[t] - 4;
+            add4(s.i0, s.i1, s.i2, s.i3, kr, (uint32_t)G.map[kr][c] * G.reg_mul[t]);
+            add4(s.j0, s.j1, s.j2, s.j3, kf, (uint32_t)G.map[kf + 4][c] * G.ref_mul[t]);
+        }
+    }
+    int a0 = pdb_read_k(P, 0, s.i0), a1 = pdb_read_k(P, 1, s.i1), a2 = pdb_read_k(P, 2, s.i2),
+        a3 = pdb_read_k(P, 3, s.i3);
+    int b0 = pdb_read_k(P, 0, s.j0), b1 = pdb_read_k(P, 1, s.j1), b2 = pdb_read_k(P, 2, s.j2),
+        b3 = pdb_read_k(P, 3, s.j3);
+    s.v0 = (uint32_t)a0 | ((uint32_t)a1 << 8) | ((uint32_t)a2 << 16) | ((uint32_t)a3 << 24);
+    s.v1 = (uint32_t)b0 | ((uint32_t)b1 << 8) | ((uint32_t)b2 << 16) | ((uint32_t)b3 << 24);
+    s.A = a0 + a1 + a2 + a3;
+    s.B = b0 + b1 + b2 + b3;
+    s.h = s.A > s.B ? s.A : s.B;
+}
+
+// Lane position: board + blank (cell, row, col) + heuristic state.
+template <int W, int HEUR> struct Pos {
+    Board<W> bd;
+    int br, bc;
+    HState<HEUR> hs;
+};
+
+// Move the blank one cell in direction d: the tile at the neighbour slides
+// into the blank.
+template <int W, int HEUR>
+__device__ __forceinline__ void do_move(Pos<W, HEUR> &s, int d, const PdbCtx &P) {
+    int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
+    int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
+    int from = nr * W + nc, to = s.br * W + s.bc;
+    int t = s.bd.tile(from);
+    if constexpr (HEUR == 0) {
+        int tr = trow<W>(t), tc = tcol<W>(t);
+        s.hs.h += abs(s.br - tr) + abs(s.bc - tc) - abs(nr - tr) - abs(nc - tc);
+    } else {
+        pdb_apply(P, s.hs, t, from, to);
+    }
+    s.bd.slide(t, from, to);
+    s.br = nr;
+    s.bc = nc;
+}
+
+template <int W, int HEUR>
+__device__ __forceinline__ void init_heur(Pos<W, HEUR> &s, const PdbCtx &P, uint32_t meta) {
+    if constexpr (HEUR == 0) {
+        s.hs.h = meta_h(meta);
+    } else {
+        pdb_init(P, s.hs, s.bd);
+    }
+}
+
+} // namespace spz

@@ -1,0 +1,90 @@
+// spz_internal.h -- device-side data layout shared by the search kernels,
+// the PDB builder and the host driver of libspz.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spz {
+
+// Blank-move directions in the reference's trial order: up, right, left,
+// down (src/24puzzle.cc:23-26, order = {1,0,2,3}); reverse(d) = 3 - d
+// (src_horie/idas_cpu.c:7-12).
+enum : int { DIR_UP = 0, DIR_RIGHT = 1, DIR_LEFT = 2, DIR_DOWN = 3, DIR_NONE = 4 };
+
+// Lane states of the DFS engine.
+enum : int { ST_IDLE = 0, ST_FRESH = 1, ST_POP = 2, ST_DRAINED = 3 };
+
+constexpr int kMaxIters = 128;      // threshold iterations kept per instance
+constexpr int kFBins = 256;         // f-value histogram bins of the top tree
+constexpr int kNoGoal = 0x7fffffff; // found-depth sentinel
+
+// A search node in the work queue (32 bytes).  The board is packed:
+//  width 4: b[0] = row0 | row1<<16, b[1] = row2 | row3<<16 (4-bit cells)
+//  width 5: b[r] = row r (25 bits) | 7 bits of row 4 at bit 25 (r = 0..3)
+// meta = blank | g<<8 | h<<16 | prev<<24; x0/x1 carry the eight PDB lookup
+// values (6 bits each) so that a lane starting the item needs no gathers.
+struct alignas(16) Item {
+    uint32_t b[4];
+    uint32_t meta;
+    uint32_t inst;
+    uint32_t x0, x1;
+};
+
+__host__ __device__ inline uint32_t meta_pack(int blank, int g, int h, int prev) {
+    return (uint32_t)blank | ((uint32_t)g << 8) | ((uint32_t)h << 16) | ((uint32_t)prev << 24);
+}
+__host__ __device__ inline int meta_blank(uint32_t m) { return (int)(m & 0xff); }
+__host__ __device__ inline int meta_g(uint32_t m) { return (int)((m >> 8) & 0xff); }
+__host__ __device__ inline int meta_h(uint32_t m) { return (int)((m >> 16) & 0xff); }
+__host__ __device__ inline int meta_prev(uint32_t m) { return (int)(m >> 24); }
+
+// Everything the kernels touch, passed by value.
+struct KParams {
+    // work queue: ring of qcap (power of two) items; q_ctl = head, limit, tail
+    Item *q;
+    unsigned long long qmask;
+    unsigned long long *q_ctl;
+
+    // instances
+    int n_inst;
+    int max_iters;
+    int mode;
+    const Item *roots;           // [n_inst]
+    int *L;                      // current threshold
+    int *it;                     // current iteration index
+    unsigned long long *cnt;     // count of the current iteration
+    unsigned int *pend;          // items of the current iteration not finished
+    int *found;                  // min goal depth (kNoGoal if none)
+    int *done;
+    int *opt;
+    unsigned long long *iter_counts; // [n_inst][max_iters]
+    int *restart_list;
+    unsigned int *ctl;           // 0 n_restart, 1 active, 2 busy, 3 error,
+                                 // 4 donate_per_lane, 5 budget, 6 rounds,
+                                 // 7 donated, 8 overflow exports
+
+    // top tree of the root split (per instance)
+    const Item *frontier;        // [n_inst][capl]
+    const unsigned int *nfront;  // [n_inst]
+    const unsigned int *topgen;  // [n_inst][kFBins] generated children by f
+    const int *top_goal;         // [n_inst] min depth of a goal in the top tree
+    int capl;
+
+    // lane contexts, structure of arrays over n_lanes lanes
+    int n_lanes;
+    uint32_t *c_st;    // state | sp<<8
+    uint32_t *c_rows;  // [5][n_lanes]
+    uint32_t *c_root;  // [4][n_lanes] packed item root
+    uint32_t *c_meta;  // blank,g,h,prev of the current node
+    uint32_t *c_rmeta; // meta of the item root
+    uint32_t *c_inst;
+    uint32_t *c_cnt;
+    uint32_t *c_pdb;   // [10][n_lanes]: 8 lookup indices + 2 packed value words
+    uint8_t *c_stack;  // [maxd][n_lanes]
+
+    // pattern databases: two canonical tables, direct 25^6 addressing
+    const uint8_t *pdb0;
+    const uint8_t *pdb1;
+};
+
+} // namespace spz

@@ -1,0 +1,668 @@
+// spz_search.hip -- the IDA* hot path on CDNA4 (gfx950).
+//
+// One search = many instances (a batch of boards), each running its own
+// threshold sequence h0, h0+2, ... (src/24puzzle.cc:146-152).  An iteration
+// of an instance is:
+//   * its top tree (root split, src/15puzzle_block_parallel.cu:150-187, done
+//     once per solve by k_split_level): the count of every top node whose f
+//     fits the threshold is added from an f-histogram, and
+//   * one work item per frontier node whose f fits the threshold; items are
+//     searched depth-first by single lanes (k_round), which may donate their
+//     shallowest unexplored children back to the queue when lanes go idle
+//     (device-wide work stealing; the reference steals only inside a block,
+//     src_horie/idas_dynamic.cu:199-247).
+// The count of iteration L is  sum_{top u: f(u)<=L} gen(u)
+//                            + sum_{frontier v: f(v)<=L} subtree_count(v),
+// where gen(u) counts every in-bounds non-reverse child of an expanded node
+// before its f-test (src/24puzzle.cc:119-131).  f is monotone along a path
+// (Manhattan and the PDB max are consistent), so a node is expanded in
+// iteration L iff its own f <= L, whatever the schedule: counts are exact.
+#include "spz_board.h"
+
+namespace spz {
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    uint32_t lo = __builtin_amdgcn_mbcnt_lo(~0u, 0u);
+    uint32_t n = __builtin_amdgcn_mbcnt_hi(~0u, lo);
+    return n == 0 ? 0ull : (~0ull >> (64 - n));
+}
+
+__device__ __forceinline__ unsigned long long shfl_u64(unsigned long long v, int src) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    lo = __shfl(lo, src);
+    hi = __shfl(hi, src);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Iteration bookkeeping when the last item of an instance's iteration ends.
+// Called by exactly one lane (the one whose pend decrement reached zero).
+__device__ void finish_iteration(const KParams &p, int inst) {
+    int it = p.it[inst];
+    unsigned long long c = atomicAdd(&p.cnt[inst], 0ull);
+    p.iter_counts[(size_t)inst * p.max_iters + it] = c;
+    int fd = atomicAdd(&p.found[inst], 0);
+    if (fd != kNoGoal) {
+        p.opt[inst] = fd;
+        p.done[inst] = 1;
+        atomicSub(&p.ctl[1], 1u);
+    } else if (it + 1 >= p.max_iters) {
+        p.opt[inst] = -1;
+        p.done[inst] = 1;
+        atomicSub(&p.ctl[1], 1u);
+    } else {
+        p.L[inst] += 2;
+        p.it[inst] = it + 1;
+        atomicExch(&p.cnt[inst], 0ull);
+        unsigned k = atomicAdd(&p.ctl[0], 1u);
+        p.restart_list[k] = inst;
+    }
+}
+
+// Append one item.  pos is a reserved queue position.
+template <int W, int HEUR>
+__device__ __forceinline__ void put_item(const KParams &p, unsigned long long pos, const Pos<W, HEUR> &s, int g,
+                                         int prev, int inst) {
+    Item it;
+    s.bd.pack(it.b);
+    it.meta = meta_pack(s.br * W + s.bc, g, s.hs.h, prev);
+    it.inst = (uint32_t)inst;
+    it.x0 = 0;
+    it.x1 = 0;
+    p.q[pos & p.qmask] = it;
+}
+
+template <int W, int HEUR>
+__device__ __forceinline__ void pos_from_item(Pos<W, HEUR> &s, const uint32_t (&b)[4], uint32_t meta,
+                                              const PdbCtx &P) {
+    s.bd.unpack(b);
+    int blank = meta_blank(meta);
+    s.br = blank / W;
+    s.bc = blank - s.br * W;
+    init_heur<W, HEUR>(s, P, meta);
+}
+
+// ---------------------------------------------------------------------------
+// k_round: every lane is an independent DFS engine.  A round runs `budget`
+// wave-uniform steps; each step a lane either takes an item, pops one stack
+// level, or expands the node it stands on.  Busy lanes keep their state
+// across rounds in the c_* context arrays.  The stack holds one byte per
+// level: the move taken from that level (2 bits) and the moves still to
+// try there (4-bit mask of children that passed the f-test).
+template <int W, int HEUR>
+__global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
+    constexpr int MAXD = Geo<W>::MAXD;
+    extern __shared__ uint8_t lds[];
+    __shared__ PdbGeo geo_s;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    const int gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const int NL = p.n_lanes;
+    uint8_t *stk = lds + wave * 64 + lane; // level k at stk[k * sstride]
+    const int sstride = nw * 64;
+
+    PdbCtx P{&geo_s, p.pdb0, p.pdb1};
+    if constexpr (HEUR == 1) {
+        const uint32_t *src = (const uint32_t *)geo_g;
+        uint32_t *dst = (uint32_t *)&geo_s;
+        for (int i = threadIdx.x; i < (int)(sizeof(PdbGeo) / 4); i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
+
+    const unsigned long long limit = p.q_ctl[1];
+    const int budget = (int)p.ctl[5];
+    const int mode = p.mode;
+
+    uint32_t stw = p.c_st[gl];
+    int st = (int)(stw & 0xff);
+    int sp = (int)(stw >> 8);
+    Pos<W, HEUR> s;
+    s.bd.unpack({0u, 0u, 0u, 0u});
+    s.br = s.bc = 0;
+    s.hs.h = 0;
+    uint32_t root[4] = {0, 0, 0, 0};
+    uint32_t rmeta = 0;
+    int g = 0, prev = DIR_NONE, inst = 0, L = 0;
+    uint32_t cnt = 0;
+
+    if (st == ST_FRESH || st == ST_POP) {
+        uint32_t b[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) b[k] = p.c_rows[k * NL + gl];
+        s.bd.unpack(b);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) root[k] = p.c_root[k * NL + gl];
+        uint32_t m = p.c_meta[gl];
+        rmeta = p.c_rmeta[gl];
+        inst = (int)p.c_inst[gl];
+        cnt = p.c_cnt[gl];
+        int blank = meta_blank(m);
+        s.br = blank / W;
+        s.bc = blank - s.br * W;
+        g = meta_g(m);
+        prev = meta_prev(m);
+        s.hs.h = meta_h(m);
+        if constexpr (HEUR == 1) {
+            s.hs.i0 = p.c_pdb[0 * NL + gl];
+            s.hs.i1 = p.c_pdb[1 * NL + gl];
+            s.hs.i2 = p.c_pdb[2 * NL + gl];
+            s.hs.i3 = p.c_pdb[3 * NL + gl];
+            s.hs.j0 = p.c_pdb[4 * NL + gl];
+            s.hs.j1 = p.c_pdb[5 * NL + gl];
+            s.hs.j2 = p.c_pdb[6 * NL + gl];
+            s.hs.j3 = p.c_pdb[7 * NL + gl];
+            s.hs.v0 = p.c_pdb[8 * NL + gl];
+            s.hs.v1 = p.c_pdb[9 * NL + gl];
+            uint32_t v0 = s.hs.v0, v1 = s.hs.v1;
+            s.hs.A = (int)((v0 & 0xff) + ((v0 >> 8) & 0xff) + ((v0 >> 16) & 0xff) + (v0 >> 24));
+            s.hs.B = (int)((v1 & 0xff) + ((v1 >> 8) & 0xff) + ((v1 >> 16) & 0xff) + (v1 >> 24));
+        }
+        L = p.L[inst];
+        for (int k = 0; k < sp; ++k) stk[k * sstride] = p.c_stack[(size_t)k * NL + gl];
+    } else {
+        st = ST_IDLE;
+    }
+
+    for (int step = 0; step < budget; ++step) {
+        // ---- take items for idle lanes (one atomic per wave) ----
+        uint64_t need = __ballot(st == ST_IDLE);
+        if (need) {
+            int n = __popcll(need);
+            int leader = __ffsll((long long)need) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&p.q_ctl[0], (unsigned long long)n);
+            base = shfl_u64(base, leader);
+            if (st == ST_IDLE) {
+                unsigned long long my = base + __popcll(need & lanemask_lt());
+                if (my < limit) {
+                    const Item it = p.q[my & p.qmask];
+                    pos_from_item<W, HEUR>(s, it.b, it.meta, P);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) root[k] = it.b[k];
+                    rmeta = it.meta;
+                    inst = (int)it.inst;
+                    g = meta_g(it.meta);
+                    prev = meta_prev(it.meta);
+                    L = p.L[inst];
+                    sp = 0;
+                    cnt = 0;
+                    st = ST_FRESH;
+                    if (mode == 1 && p.found[inst] != kNoGoal) st = ST_POP; // first-solution: abandon
+                } else {
+                    st = ST_DRAINED;
+                }
+            }
+        }
+        if (!__ballot(st == ST_FRESH || st == ST_POP)) break;
+
+        // ---- pop one level ----
+        bool finish = false;
+        if (st == ST_POP) {
+            if (sp == 0) {
+                finish = true;
+            } else {
+                uint32_t e = stk[(sp - 1) * sstride];
+                int d = (int)(e & 3u);
+                uint32_t R = e >> 2;
+                do_move<W, HEUR>(s, 3 - d, P);
+                --g;
+                if (R) {
+                    int d2 = __builtin_ctz(R);
+                    R &= R - 1;
+                    stk[(sp - 1) * sstride] = (uint8_t)((R << 2) | (uint32_t)d2);
+                    do_move<W, HEUR>(s, d2, P);
+                    ++g;
+                    prev = d2;
+                    st = ST_FRESH;
+                } else {
+                    --sp;
+                }
+            }
+        }
+        if (finish) {
+            if (cnt) atomicAdd(&p.cnt[inst], (unsigned long long)cnt);
+            __threadfence();
+            unsigned old = atomicSub(&p.pend[inst], 1u);
+            if (old == 1u) finish_iteration(p, inst);
+            cnt = 0;
+            st = ST_IDLE;
+        }
+
+        // ---- expand ----
+        if (st == ST_FRESH) {
+            const int h = s.hs.h;
+            bool expand = true;
+            if (h == 0) {
+                atomicMin(&p.found[inst], g);
+                if (mode == 1) {
+                    st = ST_POP;
+                    expand = false;
+                }
+            }
+            if (expand) {
+                const int br = s.br, bc = s.bc;
+                uint32_t legal = (br > 0 ? 1u : 0u) | (bc < W - 1 ? 2u : 0u) | (bc > 0 ? 4u : 0u) |
+                                 (br < W - 1 ? 8u : 0u);
+                uint32_t gen = legal & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
+                cnt += (uint32_t)__builtin_popcount(gen);
+                int slack = L - g - h;
+                uint32_t pass = gen;
+                if (slack < 2) {
+                    // only children with h-1 survive: h changes by exactly +-1
+                    // per move and f keeps the parity of the threshold
+                    uint32_t imp = 0;
+                    const int bl = br * W + bc;
+                    if constexpr (HEUR == 0) {
+                        int tU = s.bd.tile(br > 0 ? bl - W : bl);
+                        int tD = s.bd.tile(br < W - 1 ? bl + W : bl);
+                        int tL = s.bd.tile(bc > 0 ? bl - 1 : bl);
+                        int tR = s.bd.tile(bc < W - 1 ? bl + 1 : bl);
+                        imp |= (trow<W>(tU) >= br) ? 1u : 0u;
+                        imp |= (tcol<W>(tR) <= bc) ? 2u : 0u;
+                        imp |= (tcol<W>(tL) >= bc) ? 4u : 0u;
+                        imp |= (trow<W>(tD) <= br) ? 8u : 0u;
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < 4; ++d) {
+                            if (gen & (1u << d)) {
+                                int from = bl + (d == DIR_UP ? -W : (d == DIR_DOWN ? W : (d == DIR_LEFT ? -1 : 1)));
+                                int t = s.bd.tile(from);
+                                int hc = pdb_child_h(P, s.hs, t, from, bl);
+                                imp |= (hc < h) ? (1u << d) : 0u;
+                            }
+                        }
+                    }
+                    pass = gen & imp;
+                }
+                if (pass) {
+                    if (sp < MAXD) {
+                        int d = __builtin_ctz(pass);
+                        uint32_t R = pass & (pass - 1);
+                        stk[sp * sstride] = (uint8_t)((R << 2) | (uint32_t)d);
+                        ++sp;
+                        do_move<W, HEUR>(s, d, P);
+                        ++g;
+                        prev = d;
+                    } else {
+                        // stack full: the passing children become queue items
+                        int k = __builtin_popcount(pass);
+                        atomicAdd(&p.pend[inst], (unsigned)k);
+                        unsigned long long pos = atomicAdd(&p.q_ctl[2], (unsigned long long)k);
+                        atomicAdd(&p.ctl[8], (unsigned)k);
+                        while (pass) {
+                            int d = __builtin_ctz(pass);
+                            pass &= pass - 1;
+                            Pos<W, HEUR> c = s;
+                            do_move<W, HEUR>(c, d, P);
+                            put_item<W, HEUR>(p, pos++, c, g + 1, d, inst);
+                        }
+                        st = ST_POP;
+                    }
+                } else {
+                    st = ST_POP;
+                }
+            }
+        }
+    }
+
+    // ---- round end: donate shallow open nodes, then save the context ----
+    const bool busy = (st == ST_FRESH || st == ST_POP);
+    const int per_lane = (int)p.ctl[4];
+    int want = 0;
+    if (busy && per_lane > 0) {
+        int avail = 0;
+        for (int k = 0; k < sp && avail < per_lane; ++k) avail += __builtin_popcount((uint32_t)stk[k * sstride] >> 2);
+        want = avail < per_lane ? avail : per_lane;
+    }
+    if (__ballot(want > 0)) {
+        // wave prefix sum of `want`, one queue reservation per wave
+        int incl = want;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        int total = __shfl(incl, 63);
+        unsigned long long base = 0;
+        if (lane == 0) {
+            base = atomicAdd(&p.q_ctl[2], (unsigned long long)total);
+            atomicAdd(&p.ctl[7], (unsigned)total);
+        }
+        base = shfl_u64(base, 0);
+        if (want > 0) {
+            unsigned long long pos = base + (unsigned long long)(incl - want);
+            atomicAdd(&p.pend[inst], (unsigned)want);
+            // replay from the item root along the taken moves
+            Pos<W, HEUR> c;
+            pos_from_item<W, HEUR>(c, root, rmeta, P);
+            int cg = meta_g(rmeta);
+            int emitted = 0;
+            for (int k = 0; k < sp && emitted < want; ++k) {
+                uint32_t e = stk[k * sstride];
+                int d = (int)(e & 3u);
+                uint32_t R = e >> 2;
+                while (R && emitted < want) {
+                    int d2 = __builtin_ctz(R);
+                    R &= R - 1;
+                    Pos<W, HEUR> x = c;
+                    do_move<W, HEUR>(x, d2, P);
+                    put_item<W, HEUR>(p, pos++, x, cg + 1, d2, inst);
+                    ++emitted;
+                }
+                stk[k * sstride] = (uint8_t)((R << 2) | (uint32_t)d);
+                if (emitted < want) {
+                    do_move<W, HEUR>(c, d, P);
+                    ++cg;
+                }
+            }
+        }
+    }
+
+    uint64_t busy_mask = __ballot(busy);
+    if (lane == 0 && busy_mask) atomicAdd(&p.ctl[2], (unsigned)__popcll(busy_mask));
+    if (busy) {
+        p.c_st[gl] = (uint32_t)st | ((uint32_t)sp << 8);
+        uint32_t b[4];
+        s.bd.pack(b);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) p.c_rows[k * NL + gl] = b[k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) p.c_root[k * NL + gl] = root[k];
+        p.c_meta[gl] = meta_pack(s.br * W + s.bc, g, s.hs.h, prev);
+        p.c_rmeta[gl] = rmeta;
+        p.c_inst[gl] = (uint32_t)inst;
+        p.c_cnt[gl] = cnt;
+        if constexpr (HEUR == 1) {
+            p.c_pdb[0 * NL + gl] = s.hs.i0;
+            p.c_pdb[1 * NL + gl] = s.hs.i1;
+            p.c_pdb[2 * NL + gl] = s.hs.i2;
+            p.c_pdb[3 * NL + gl] = s.hs.i3;
+            p.c_pdb[4 * NL + gl] = s.hs.j0;
+            p.c_pdb[5 * NL + gl] = s.hs.j1;
+            p.c_pdb[6 * NL + gl] = s.hs.j2;
+            p.c_pdb[7 * NL + gl] = s.hs.j3;
+            p.c_pdb[8 * NL + gl] = s.hs.v0;
+            p.c_pdb[9 * NL + gl] = s.hs.v1;
+        }
+        for (int k = 0; k < sp; ++k) p.c_stack[(size_t)k * NL + gl] = stk[k * sstride];
+    } else {
+        p.c_st[gl] = ST_IDLE;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_restart: start the current iteration of every instance on the restart
+// list: add the top-tree count from the f-histogram, note goals in the top
+// tree, and queue the frontier nodes whose f fits.  An iteration with no
+// frontier work completes here.  One block per listed instance.
+__global__ void __launch_bounds__(256) k_restart(KParams p) {
+    __shared__ unsigned long long s_cnt;
+    __shared__ unsigned s_n;
+    __shared__ unsigned long long s_base;
+    __shared__ int s_more;
+    const unsigned n_list = p.ctl[0];
+    if (blockIdx.x >= n_list) return;
+    const int inst = p.restart_list[blockIdx.x];
+    const unsigned nf = min(p.nfront[inst], (unsigned)p.capl);
+    const Item *F = p.frontier + (size_t)inst * p.capl;
+
+    for (;;) {
+        const int L = p.L[inst];
+        if (threadIdx.x == 0) {
+            s_cnt = 0;
+            s_n = 0;
+        }
+        __syncthreads();
+        // top tree: sum of gen over f <= L
+        unsigned long long c = 0;
+        for (int f = threadIdx.x; f < kFBins; f += blockDim.x)
+            if (f <= L) c += p.topgen[(size_t)inst * kFBins + f];
+        atomicAdd(&s_cnt, c);
+        // frontier nodes that fit
+        unsigned mine = 0;
+        for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) {
+            uint32_t m = F[i].meta;
+            if (meta_g(m) + meta_h(m) <= L) ++mine;
+        }
+        atomicAdd(&s_n, mine);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_base = s_n ? atomicAdd(&p.q_ctl[2], (unsigned long long)s_n) : 0ull;
+            if (p.top_goal[inst] <= L) atomicMin(&p.found[inst], p.top_goal[inst]);
+            p.cnt[inst] = s_cnt;
+            p.pend[inst] = s_n;
+            s_n = 0; // reused as the write cursor
+        }
+        __syncthreads();
+        for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) {
+            Item it = F[i];
+            if (meta_g(it.meta) + meta_h(it.meta) <= L) {
+                unsigned k = atomicAdd(&s_n, 1u);
+                p.q[(s_base + k) & p.qmask] = it;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_more = 0;
+            if (p.pend[inst] == 0) {
+                // nothing to search below the split: the iteration is complete
+                int it = p.it[inst];
+                p.iter_counts[(size_t)inst * p.max_iters + it] = s_cnt;
+                if (p.found[inst] != kNoGoal) {
+                    p.opt[inst] = p.found[inst];
+                    p.done[inst] = 1;
+                    atomicSub(&p.ctl[1], 1u);
+                } else if (it + 1 >= p.max_iters) {
+                    p.opt[inst] = -1;
+                    p.done[inst] = 1;
+                    atomicSub(&p.ctl[1], 1u);
+                } else {
+                    p.L[inst] = L + 2;
+                    p.it[inst] = it + 1;
+                    s_more = 1;
+                }
+            }
+        }
+        __syncthreads();
+        if (!s_more) break;
+    }
+}
+
+// k_control: between rounds.  Publishes the items appended during the last
+// round, sizes the donation quota from the number of idle lanes, and picks
+// the next round's step budget.  ctl[9] = budget used when the queue can keep
+// every lane busy, ctl[10] = budget while it cannot.
+__global__ void k_control(KParams p) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned long long head = p.q_ctl[0], limit = p.q_ctl[1], tail = p.q_ctl[2];
+    if (head > limit) head = limit;
+    p.q_ctl[0] = head;
+    p.q_ctl[1] = tail;
+    unsigned long long qlen = tail - head;
+    unsigned busy = p.ctl[2];
+    p.ctl[2] = 0;
+    p.ctl[0] = 0; // restart list consumed
+    unsigned long long lanes = (unsigned long long)p.n_lanes;
+    unsigned long long idle = lanes > busy ? lanes - busy : 0ull;
+    unsigned per_lane = 0;
+    if (qlen < idle && busy > 0) {
+        unsigned long long want = idle - qlen;
+        per_lane = (unsigned)((want + busy - 1) / busy);
+        // donations of one round stay below a quarter of the ring
+        unsigned long long cap = ((p.qmask + 1ull) / 4ull) / (unsigned long long)busy;
+        if (per_lane > cap) per_lane = (unsigned)cap;
+        if (per_lane > 32u) per_lane = 32u;
+    }
+    p.ctl[4] = per_lane;
+    p.ctl[5] = (qlen + busy >= lanes / 2) ? p.ctl[9] : p.ctl[10];
+    p.ctl[6] += 1;
+}
+
+// ---------------------------------------------------------------------------
+// k_split_level: one level of the root split, all instances at once.  A level
+// that already holds >= target nodes (or is at max depth) becomes the
+// instance's frontier; otherwise every node is expanded without pruning:
+// its generated children are counted into the top-tree histogram under the
+// node's f, and the children form the next level.
+template <int W, int HEUR>
+__global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *geo_g, const Item *lvl_in,
+                                                     const unsigned *n_in, Item *lvl_out, unsigned *n_out,
+                                                     unsigned *nfront, unsigned *topgen, int *top_goal,
+                                                     int *frozen, int depth, int max_depth, unsigned target) {
+    __shared__ PdbGeo geo_s;
+    PdbCtx P{&geo_s, p.pdb0, p.pdb1};
+    if constexpr (HEUR == 1) {
+        const uint32_t *src = (const uint32_t *)geo_g;
+        uint32_t *dst = (uint32_t *)&geo_s;
+        for (int i = threadIdx.x; i < (int)(sizeof(PdbGeo) / 4); i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int inst = (int)(gid / (size_t)p.capl);
+    const unsigned j = (unsigned)(gid % (size_t)p.capl);
+    if (inst >= p.n_inst) return;
+    if (frozen[inst]) return;
+    const unsigned n = min(n_in[inst], (unsigned)p.capl);
+    const bool freeze = (n >= target) || (depth >= max_depth);
+    if (freeze) {
+        if (j < n) ((Item *)p.frontier)[(size_t)inst * p.capl + j] = lvl_in[(size_t)inst * p.capl + j];
+        if (j == 0) {
+            nfront[inst] = n;
+            frozen[inst] = 1;
+        }
+        return;
+    }
+    if (j >= n) return;
+    Item it = lvl_in[(size_t)inst * p.capl + j];
+    Pos<W, HEUR> s;
+    pos_from_item<W, HEUR>(s, it.b, it.meta, P);
+    int g = meta_g(it.meta), prev = meta_prev(it.meta);
+    const int br = s.br, bc = s.bc;
+    int f = g + s.hs.h;
+    if (s.hs.h == 0) atomicMin(&top_goal[inst], g);
+    uint32_t legal = (br > 0 ? 1u : 0u) | (bc < W - 1 ? 2u : 0u) | (bc > 0 ? 4u : 0u) | (br < W - 1 ? 8u : 0u);
+    uint32_t gen = legal & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
+    atomicAdd(&topgen[(size_t)inst * kFBins + (f < kFBins ? f : kFBins - 1)], (unsigned)__builtin_popcount(gen));
+    unsigned k = atomicAdd(&n_out[inst], (unsigned)__builtin_popcount(gen));
+    while (gen) {
+        int d = __builtin_ctz(gen);
+        gen &= gen - 1;
+        Pos<W, HEUR> c = s;
+        do_move<W, HEUR>(c, d, P);
+        Item ci;
+        c.bd.pack(ci.b);
+        ci.meta = meta_pack(c.br * W + c.bc, g + 1, c.hs.h, d);
+        ci.inst = (uint32_t)inst;
+        ci.x0 = 0;
+        ci.x1 = 0;
+        if (k < (unsigned)p.capl) lvl_out[(size_t)inst * p.capl + k] = ci;
+        ++k;
+    }
+}
+
+// heuristic of packed items (spz_eval_h and the root h0)
+template <int W, int HEUR>
+__global__ void k_eval_h(KParams p, const PdbGeo *geo_g, Item *items, int n) {
+    __shared__ PdbGeo geo_s;
+    PdbCtx P{&geo_s, p.pdb0, p.pdb1};
+    if constexpr (HEUR == 1) {
+        const uint32_t *src = (const uint32_t *)geo_g;
+        uint32_t *dst = (uint32_t *)&geo_s;
+        for (int i = threadIdx.x; i < (int)(sizeof(PdbGeo) / 4); i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Item it = items[i];
+    Board<W> bd;
+    bd.unpack(it.b);
+    int h = 0;
+    if constexpr (HEUR == 0) {
+        for (int c = 0; c < W * W; ++c) {
+            int t = bd.tile(c);
+            if (t) h += abs(c / W - trow<W>(t)) + abs(c % W - tcol<W>(t));
+        }
+    } else {
+        HState<1> hs;
+        pdb_init(P, hs, bd);
+        h = hs.h;
+    }
+    it.meta = (it.meta & 0xff00ffffu) | ((uint32_t)h << 16);
+    items[i] = it;
+}
+
+// roots -> slot 0 of each instance's level-0 region
+__global__ void k_scatter_roots(const Item *roots, Item *lvl, int capl, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) lvl[(size_t)i * capl] = roots[i];
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+template <int W, int HEUR>
+static hipError_t launch_round_t(const KParams &p, const PdbGeo *geo, int wg, hipStream_t s) {
+    size_t lds = (size_t)Geo<W>::MAXD * wg;
+    hipLaunchKernelGGL((k_round<W, HEUR>), dim3(p.n_lanes / wg), dim3(wg), lds, s, p, geo);
+    return hipGetLastError();
+}
+
+hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo, int wg, hipStream_t s) {
+    if (width == 4) return launch_round_t<4, 0>(p, geo, wg, s);
+    if (heur == 0) return launch_round_t<5, 0>(p, geo, wg, s);
+    return launch_round_t<5, 1>(p, geo, wg, s);
+}
+
+size_t round_lds_bytes(int width, int wg) { return (size_t)(width == 4 ? Geo<4>::MAXD : Geo<5>::MAXD) * wg; }
+
+const void *round_kernel_ptr(int width, int heur) {
+    if (width == 4) return (const void *)&k_round<4, 0>;
+    if (heur == 0) return (const void *)&k_round<5, 0>;
+    return (const void *)&k_round<5, 1>;
+}
+
+hipError_t launch_restart(const KParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_restart, dim3(p.n_inst), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_control(const KParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_control, dim3(1), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_level(int width, int heur, const KParams &p, const PdbGeo *geo, const Item *lvl_in,
+                              const unsigned *n_in, Item *lvl_out, unsigned *n_out, unsigned *nfront,
+                              unsigned *topgen, int *top_goal, int *frozen, int depth, int max_depth,
+                              unsigned target, hipStream_t s) {
+    size_t total = (size_t)p.n_inst * p.capl;
+    dim3 grid((unsigned)((total + 255) / 256));
+    if (width == 4)
+        hipLaunchKernelGGL((k_split_level<4, 0>), grid, dim3(256), 0, s, p, geo, lvl_in, n_in, lvl_out, n_out, nfront,
+                           topgen, top_goal, frozen, depth, max_depth, target);
+    else if (heur == 0)
+        hipLaunchKernelGGL((k_split_level<5, 0>), grid, dim3(256), 0, s, p, geo, lvl_in, n_in, lvl_out, n_out, nfront,
+                           topgen, top_goal, frozen, depth, max_depth, target);
+    else
+        hipLaunchKernelGGL((k_split_level<5, 1>), grid, dim3(256), 0, s, p, geo, lvl_in, n_in, lvl_out, n_out, nfront,
+                           topgen, top_goal, frozen, depth, max_depth, target);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_roots(const Item *roots, Item *lvl, int capl, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_roots, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, roots, lvl, capl, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_h(int width, int heur, const KParams &p, const PdbGeo *geo, Item *items, int n,
+                         hipStream_t s) {
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (width == 4)
+        hipLaunchKernelGGL((k_eval_h<4, 0>), grid, dim3(256), 0, s, p, geo, items, n);
+    else if (heur == 0)
+        hipLaunchKernelGGL((k_eval_h<5, 0>), grid, dim3(256), 0, s, p, geo, items, n);
+    else
+        hipLaunchKernelGGL((k_eval_h<5, 1>), grid, dim3(256), 0, s, p, geo, items, n);
+    return hipGetLastError();
+}
+
+} // namespace spz

@@ -1,0 +1,250 @@
+"""spz -- Python binding of libspz, the MI355X IDA* solver (ctypes over the C ABI
+declared in include/spz.h).
+
+The reference is a set of solver processes: read a problem file of N*N
+integers (0 = blank), run IDA* with thresholds h0, h0+2, ..., print
+"<path> <len> <nodes>" (src/24puzzle.cc:72-87,146-198).  This module offers
+the same surface for a whole batch on the GPU:
+
+    with Solver(width=4) as s:
+        res = s.solve(boards)            # list of Result
+        res[0].opt_len, res[0].counts    # per-iteration node counts
+
+The native library must be built (`python sliding-puzzle_amd/build.py`);
+there is no CPU fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libspz.so")
+
+H_MANHATTAN, H_PDB666 = 0, 1
+MODE_ALL, MODE_FIRST = 0, 1
+LB_UNIFORM, LB_LEARNED = 0, 1
+MAX_ITERS = 128
+
+ERRORS = {-1: "bad argument", -2: "HIP error", -3: "no PDB", -4: "queue capacity",
+          -5: "I/O error", -6: "unsolvable", -7: "stalled", -8: "timeout"}
+
+
+class SpzError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("heuristic", ctypes.c_int32),
+                ("mode", ctypes.c_int32), ("frontier_target", ctypes.c_int32),
+                ("max_split_depth", ctypes.c_int32), ("workgroup_size", ctypes.c_int32),
+                ("lanes", ctypes.c_int32), ("round_budget", ctypes.c_int32),
+                ("lb_policy", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("queue_items", ctypes.c_int64)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("opt_len", ctypes.c_int32), ("h0", ctypes.c_int32),
+                ("n_iters", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("total_nodes", ctypes.c_uint64)]
+
+
+@dataclass
+class Result:
+    opt_len: int
+    h0: int
+    n_iters: int
+    status: int
+    total_nodes: int
+    thresholds: List[int] = field(default_factory=list)
+    counts: List[int] = field(default_factory=list)
+
+
+_lib = None
+
+# every entry point declared in include/spz.h
+EXPORTS = ["spz_default_config", "spz_create", "spz_build_pdb", "spz_export_pdb_tab",
+           "spz_get_pdb", "spz_solve_batch", "spz_eval_h", "spz_set_lb", "spz_last_stats",
+           "spz_last_error", "spz_destroy", "spz_pdb_geometry", "spz_gen_walk",
+           "spz_gen_random"]
+
+
+def lib() -> ctypes.CDLL:
+    """Load libspz.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SpzError(f"{LIB_PATH} is missing: build it with sliding-puzzle_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u8p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8)
+    L.spz_default_config.argtypes = [ctypes.POINTER(Config), ctypes.c_int32, ctypes.c_int32]
+    L.spz_default_config.restype = None
+    L.spz_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Config)]
+    L.spz_build_pdb.argtypes = [vp, ctypes.c_char_p]
+    L.spz_export_pdb_tab.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p]
+    L.spz_get_pdb.argtypes = [vp, ctypes.c_int32, u8p]
+    L.spz_solve_batch.argtypes = [vp, u8p, ctypes.c_int32, ctypes.c_int32,
+                                  ctypes.POINTER(_Result), ctypes.POINTER(ctypes.c_uint64)]
+    L.spz_eval_h.argtypes = [vp, u8p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    L.spz_set_lb.argtypes = [vp, ctypes.c_int32]
+    L.spz_last_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.spz_last_error.argtypes = [vp]
+    L.spz_last_error.restype = ctypes.c_char_p
+    L.spz_destroy.argtypes = [vp]
+    L.spz_destroy.restype = None
+    L.spz_pdb_geometry.argtypes = [u8p, u8p]
+    L.spz_pdb_geometry.restype = None
+    L.spz_gen_walk.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, u8p]
+    L.spz_gen_walk.restype = None
+    L.spz_gen_random.argtypes = [ctypes.c_int32, ctypes.c_uint64, u8p]
+    L.spz_gen_random.restype = None
+    _lib = L
+    return L
+
+
+def _u8p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def gen_walk(width: int, steps: int, seed: int) -> np.ndarray:
+    out = np.zeros(width * width, dtype=np.uint8)
+    lib().spz_gen_walk(width, steps, seed, _u8p(out))
+    return out
+
+
+def gen_random(width: int, seed: int) -> np.ndarray:
+    out = np.zeros(width * width, dtype=np.uint8)
+    lib().spz_gen_random(width, seed, _u8p(out))
+    return out
+
+
+def pdb_geometry():
+    tiles = np.zeros(48, dtype=np.uint8)
+    maps = np.zeros(200, dtype=np.uint8)
+    lib().spz_pdb_geometry(_u8p(tiles), _u8p(maps))
+    return tiles.reshape(8, 6), maps.reshape(8, 25)
+
+
+def read_problem(path: str, width: Optional[int] = None) -> np.ndarray:
+    """Parse a reference problem file: N*N whitespace-separated integers,
+    0 = blank (src/24puzzle.cc:72-87).  The width is inferred when omitted."""
+    with open(path) as f:
+        vals = [int(x) for x in f.read().split()]
+    if width is None:
+        width = 5 if len(vals) >= 25 else 4
+    n2 = width * width
+    if len(vals) < n2:
+        raise SpzError(f"{path}: expected {n2} integers, found {len(vals)}")
+    return np.array(vals[:n2], dtype=np.uint8)
+
+
+def format_line(path: str, res: Result) -> str:
+    """The reference's result line: "<path> <len> <cumulative nodes>"
+    (src/24puzzle_expand.cc:154, src/24puzzle_expand_with_pdb.cpp)."""
+    return f"{path} {res.opt_len} {res.total_nodes}"
+
+
+class Solver:
+    """One device context (see spz_create).  Not thread-safe."""
+
+    def __init__(self, width: int = 4, heuristic: int = H_MANHATTAN, mode: int = MODE_ALL,
+                 frontier_target: int = 2048, max_split_depth: int = 24,
+                 workgroup_size: int = 256, lanes: int = 0, round_budget: int = 0,
+                 lb_policy: int = LB_UNIFORM, device: int = 0, queue_items: int = 0):
+        L = lib()
+        cfg = Config()
+        L.spz_default_config(ctypes.byref(cfg), width, heuristic)
+        cfg.mode = mode
+        cfg.frontier_target = frontier_target
+        cfg.max_split_depth = max_split_depth
+        cfg.workgroup_size = workgroup_size
+        cfg.lanes = lanes
+        cfg.round_budget = round_budget
+        cfg.lb_policy = lb_policy
+        cfg.device = device
+        cfg.queue_items = queue_items
+        self.cfg = cfg
+        self.width = width
+        self._ctx = ctypes.c_void_p()
+        rc = L.spz_create(ctypes.byref(self._ctx), ctypes.byref(cfg))
+        if rc != 0:
+            msg = L.spz_last_error(self._ctx).decode() if self._ctx else ERRORS.get(rc, str(rc))
+            if self._ctx:
+                L.spz_destroy(self._ctx)
+            self._ctx = None
+            raise SpzError(f"spz_create failed ({rc}): {msg}")
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = lib().spz_last_error(self._ctx).decode()
+            raise SpzError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}")
+
+    def close(self):
+        if self._ctx:
+            lib().spz_destroy(self._ctx)
+            self._ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build_pdb(self, cache_path: Optional[str] = None):
+        self._check(lib().spz_build_pdb(self._ctx, cache_path.encode() if cache_path else None),
+                    "spz_build_pdb")
+
+    def export_pdb_tab(self, path0: str, path1: str):
+        self._check(lib().spz_export_pdb_tab(self._ctx, path0.encode(), path1.encode()),
+                    "spz_export_pdb_tab")
+
+    def get_pdb(self, pattern: int) -> np.ndarray:
+        out = np.empty(127_512_000, dtype=np.uint8)
+        self._check(lib().spz_get_pdb(self._ctx, pattern, _u8p(out)), "spz_get_pdb")
+        return out
+
+    def set_lb(self, policy: int):
+        self._check(lib().spz_set_lb(self._ctx, policy), "spz_set_lb")
+
+    def eval_h(self, boards) -> np.ndarray:
+        b = np.ascontiguousarray(np.asarray(boards, dtype=np.uint8).reshape(-1, self.width ** 2))
+        h = np.zeros(len(b), dtype=np.int32)
+        self._check(lib().spz_eval_h(self._ctx, _u8p(b), len(b),
+                                     h.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+                    "spz_eval_h")
+        return h
+
+    def solve(self, boards, max_iters: int = 64) -> List[Result]:
+        b = np.ascontiguousarray(np.asarray(boards, dtype=np.uint8).reshape(-1, self.width ** 2))
+        n = len(b)
+        out = (_Result * max(n, 1))()
+        ic = np.zeros((max(n, 1), max_iters), dtype=np.uint64)
+        self._check(lib().spz_solve_batch(self._ctx, _u8p(b), n, max_iters, out,
+                                          ic.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))),
+                    "spz_solve_batch")
+        res = []
+        for i in range(n):
+            r = out[i]
+            k = r.n_iters
+            res.append(Result(r.opt_len, r.h0, k, r.status, int(r.total_nodes),
+                              [r.h0 + 2 * j for j in range(k)],
+                              [int(x) for x in ic[i, :k]]))
+        return res
+
+    def stats(self) -> dict:
+        s = (ctypes.c_double * 8)()
+        self._check(lib().spz_last_stats(self._ctx, s), "spz_last_stats")
+        return {"rounds": s[0], "solve_ms": s[1], "donated": s[2], "lanes": s[3],
+                "pdb_levels0": s[4], "pdb_levels1": s[5], "pdb_build_ms": s[6],
+                "overflow_items": s[7]}
