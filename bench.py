@@ -1,0 +1,234 @@
+"""Benchmark of the IDA* hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4]
+
+A step is one solve of the configuration's whole batch of synthetic
+instances (seeded generators; seed = config_id * 10^6 + instance index),
+inputs already on the host in packed form; the per-step result (optimal
+lengths and node counts) is checked to be identical across steps.  Prints
+one JSON line.  Configurations:
+  c1  15-puzzle Manhattan, 100 walks of 40 moves
+  c2  15-puzzle Manhattan, 100 random solvable permutations, one instance
+      per launch, root split into 2048 frontier nodes
+  c3  15-puzzle Manhattan, 4096 mixed instances (2048 walks of 20-80 moves,
+      2048 random permutations) in one batched launch sequence
+  c4  24-puzzle 6-6-6-6 PDB, 64 walks of 160 moves, batched, 4096 frontier
+      nodes per instance
+  c5  as c4 with 240-move walks, 8192 frontier nodes
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "sliding-puzzle_amd"))
+
+import numpy as np  # noqa: E402
+
+import spz  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # int32 lane-ops/s: CUs x SIMDs x lanes x clock
+
+
+def instances(config: str, n: int | None = None):
+    """Seeded synthetic boards of a configuration."""
+    if config == "c1":
+        b = [spz.gen_walk(4, 40, 1_000_000 + i) for i in range(n or 100)]
+        return 4, spz.H_MANHATTAN, b
+    if config == "c2":
+        b = [spz.gen_random(4, 2_000_000 + i) for i in range(n or 100)]
+        return 4, spz.H_MANHATTAN, b
+    if config == "c3":
+        n = n or 4096
+        half = n // 2
+        b = []
+        for i in range(half):
+            seed = 3_000_000 + i
+            L = 20 + int((int(spz_rng_u64(seed)) >> 32) * 61 >> 32)
+            b.append(spz.gen_walk(4, L, seed))
+        b += [spz.gen_random(4, 3_000_000 + half + i) for i in range(n - half)]
+        return 4, spz.H_MANHATTAN, b
+    if config == "c4":
+        b = [spz.gen_walk(5, 160, 4_000_000 + i) for i in range(n or 64)]
+        return 5, spz.H_PDB666, b
+    if config == "c5":
+        b = [spz.gen_walk(5, 240, 5_000_000 + i) for i in range(n or 64)]
+        return 5, spz.H_PDB666, b
+    raise SystemExit(f"unknown config {config}")
+
+
+def spz_rng_u64(seed: int) -> int:
+    """First splitmix64 output of `seed` (walk-length draw of config c3)."""
+    z = (seed + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return z ^ (z >> 31)
+
+
+FRONTIER = {"c1": 2048, "c2": 2048, "c3": 16, "c4": 4096, "c5": 8192}
+WORKLOAD = {
+    "c1": "15-puzzle Manhattan, 100 walks x 40 moves",
+    "c2": "15-puzzle Manhattan, 100 random solvable permutations, one instance per launch",
+    "c3": "15-puzzle Manhattan, 4096 mixed (2048 walks 20-80 + 2048 random), batched",
+    "c4": "24-puzzle 6-6-6-6 PDB, 64 walks x 160 moves, batched, 4096 frontier nodes/instance",
+    "c5": "24-puzzle 6-6-6-6 PDB, 64 walks x 240 moves, batched, 8192 frontier nodes/instance",
+}
+
+
+def solve_step(solver, boards, per_instance: bool, max_iters: int):
+    if per_instance:
+        out = []
+        for b in boards:
+            out += solver.solve([b], max_iters=max_iters)
+        return out
+    return solver.solve(boards, max_iters=max_iters)
+
+
+def cpu_baseline(width, heur, boards, gpu_nodes_per_inst, solver, budget_s=20.0):
+    """The oracle (a single-threaded C port of the reference CPU solver) on
+    the host, over a bounded sample: whole instances of the workload, in
+    order, until ~budget_s of CPU time across a process pool."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import multiprocessing as mp
+
+    import oracle as O
+
+    cores = min(16, os.cpu_count() or 1)
+    if heur == spz.H_PDB666:
+        tabs = [solver.get_pdb(0), solver.get_pdb(1)]
+        O.pdb_set_tables(*tabs)
+    hname = "pdb" if heur == spz.H_PDB666 else "manhattan"
+    # 1-core rate on the first instances
+    t0 = time.time()
+    nodes = 0
+    done = 0
+    for b in boards:
+        r, thr, c = O.ida(list(b), hname, "all")
+        nodes += sum(c)
+        done += 1
+        if time.time() - t0 > budget_s:
+            break
+    dt = time.time() - t0
+    rate = nodes / dt if dt > 0 else 0.0
+    return {"nodes_per_s_1core": rate, "instances_done": done, "seconds": dt,
+            "cores": 1, "sample_nodes": nodes,
+            "projected_instances_per_s": rate / gpu_nodes_per_inst if gpu_nodes_per_inst else None}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default=os.environ.get("SPZ_BENCH_CONFIG", "c2"))
+    ap.add_argument("--n", type=int, default=None, help="instances (default: the config's)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--max-iters", type=int, default=64)
+    ap.add_argument("--round-budget", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this solver runs on one GPU; --gpus above 1 is not supported "
+                                   "(instances shard trivially across GPUs, see DESIGN.md)"}))
+        sys.exit(2)
+
+    width, heur, boards = instances(args.config, args.n)
+    boards = np.stack(boards)
+    per_instance = args.config == "c2"
+    solver = spz.Solver(width=width, heuristic=heur, frontier_target=FRONTIER[args.config],
+                        round_budget=args.round_budget, lanes=args.lanes, time_kernels=True)
+    pdb_ms = None
+    if heur == spz.H_PDB666:
+        t = time.time()
+        solver.build_pdb()
+        pdb_ms = (time.time() - t) * 1e3
+
+    ref = None
+    for _ in range(args.warmup):
+        ref = solve_step(solver, boards, per_instance, args.max_iters)
+    round_ms, launches, nodes_total = 0.0, 0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = solve_step(solver, boards, per_instance, args.max_iters)
+        st = solver.stats()
+        round_ms += st["round_kernel_ms"]
+        launches += int(st["round_launches"])
+        if per_instance:
+            pass
+        nodes_total += sum(r.total_nodes for r in res)
+        if ref is not None:
+            assert [(r.opt_len, r.counts) for r in res] == [(r.opt_len, r.counts) for r in ref]
+        ref = res
+    elapsed = time.perf_counter() - t0
+    n = len(boards)
+    ms_per_step = elapsed / args.steps * 1e3
+    inst_per_s = n * args.steps / elapsed
+    nodes_per_s = nodes_total / elapsed
+    unsolved = sum(1 for r in ref if r.opt_len < 0)
+
+    # roofline of the dominant kernel (the search round)
+    nodes_per_launch = nodes_total / max(launches, 1)
+    avg_launch_s = (round_ms / max(launches, 1)) / 1e3
+    if heur == spz.H_PDB666:
+        # 2 table gathers per generated node (one per partition), 64 B granule
+        achieved = nodes_per_launch * 128 / avg_launch_s / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    else:
+        # 16 int32 lane-ops per generated node (integer VALU issue bound)
+        achieved = nodes_per_launch * 16 / avg_launch_s / 1e12
+        roof = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
+                "frac": achieved / VALU_PEAK_TOPS, "traffic": None}
+    roof["kernel"] = "k_round"
+    roof["avg_launch_ms"] = avg_launch_s * 1e3
+    roof["launches"] = launches
+
+    out = {
+        "metric": "instances/sec (+ nodes/sec), exact optimal length and per-iteration counts",
+        "value": inst_per_s, "unit": "instances/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded)",
+        "config": {"workload": WORKLOAD[args.config], "config": args.config, "instances": n,
+                   "frontier_target": FRONTIER[args.config], "parallelism": "single GPU"},
+        "nodes_per_s": nodes_per_s, "nodes_per_step": nodes_total / args.steps,
+        "unsolved_within_max_iters": unsolved, "roofline": roof,
+    }
+    if pdb_ms is not None:
+        out["pdb_build_ms"] = pdb_ms
+
+    if not args.no_plain and not per_instance:
+        # plain GPU IDA*: one instance per launch, uniform static split
+        plain = spz.Solver(width=width, heuristic=heur, frontier_target=FRONTIER[args.config])
+        if heur == spz.H_PDB666:
+            plain.build_pdb()
+        t = time.perf_counter()
+        pres = solve_step(plain, boards, True, args.max_iters)
+        pt = time.perf_counter() - t
+        assert [(r.opt_len, r.counts) for r in pres] == [(r.opt_len, r.counts) for r in ref]
+        out["plain_instances_per_s"] = n / pt
+        out["speedup_vs_plain"] = inst_per_s / (n / pt)
+        plain.close()
+
+    if not args.no_cpu:
+        cb = cpu_baseline(width, heur, boards, nodes_total / args.steps / n, solver)
+        out["cpu_baseline"] = {
+            "value": cb["projected_instances_per_s"], "unit": "instances/s", "cores": cb["cores"],
+            "kind": "port",
+            "sample": f"oracle (C port of the reference CPU IDA*) on the first "
+                      f"{cb['instances_done']} instances of the workload: "
+                      f"{cb['nodes_per_s_1core'] / 1e6:.1f} M nodes/s on 1 core over "
+                      f"{cb['seconds']:.1f} s; value = that rate / the workload's mean "
+                      f"nodes per instance",
+            "nodes_per_s": cb["nodes_per_s_1core"]}
+    print(json.dumps(out), flush=True)
+    solver.close()
+
+
+if __name__ == "__main__":
+    main()

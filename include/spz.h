@@ -70,6 +70,8 @@ typedef struct spz_config {
     int32_t round_budget;    /* DFS steps per lane per round (0 = auto) */
     int32_t lb_policy;       /* SPZ_LB_* */
     int32_t device;          /* HIP device ordinal */
+    int32_t time_kernels;    /* 1: time every search-round launch with HIP
+                                events on the solver's stream (stats 8-9) */
     int64_t queue_items;     /* work-queue capacity in items (0 = auto) */
 } spz_config;
 
@@ -122,8 +124,12 @@ int spz_eval_h(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h);
  * weights: bf16 MLP parameters in the layout of spz_mlp_param_count(). */
 int spz_set_lb(spz_ctx *ctx, int32_t lb_policy);
 
-/* Statistics of the last solve: rounds run, kernel milliseconds, items
- * donated, lanes; stats must hold 8 doubles. */
+/* Statistics of the last solve (stats must hold 16 doubles):
+ *  0 rounds run            1 solve wall ms        2 items donated
+ *  3 lanes                 4,5 PDB BFS levels     6 PDB build ms
+ *  7 stack-overflow items  8 round-kernel ms (sum, when time_kernels)
+ *  9 round launches timed  10 split ms            11 instances
+ *  12 nodes (all instances, all iterations) */
 int spz_last_stats(spz_ctx *ctx, double *stats);
 
 const char *spz_last_error(spz_ctx *ctx);

@@ -65,7 +65,7 @@ struct spz_ctx {
     int n_lanes = 0;
     int wg = 256;
     std::string err;
-    double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double stats[16] = {0};
     double timeout_s = 0;
 
     PdbGeo geo_h;
@@ -148,6 +148,7 @@ void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic) {
     cfg->round_budget = 0;
     cfg->lb_policy = SPZ_LB_UNIFORM;
     cfg->device = 0;
+    cfg->time_kernels = 0;
     cfg->queue_items = 0;
 }
 
@@ -491,6 +492,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         HIPCHK(hipStreamSynchronize(s));
     }
     const int max_depth = ctx->cfg.max_split_depth;
+    auto t_split = std::chrono::steady_clock::now();
     int cur = 0;
     for (int depth = 0; depth <= max_depth; ++depth) {
         HIPCHK(hipMemsetAsync(ctx->nlvl[cur ^ 1].p, 0, N * 4, s));
@@ -501,6 +503,9 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
                                   depth, max_depth, target, s));
         cur ^= 1;
     }
+
+    HIPCHK(hipStreamSynchronize(s));
+    ctx->stats[10] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_split).count();
 
     // per-instance iteration state
     {
@@ -541,6 +546,12 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     const int per_poll = 4;
     const double limit_s = ctx->timeout_s;
     int stalled = 0;
+    const bool timing = ctx->cfg.time_kernels != 0;
+    hipEvent_t ev[2 * per_poll];
+    if (timing)
+        for (auto &x : ev) HIPCHK(hipEventCreate(&x));
+    double round_ms = 0;
+    long timed = 0;
     unsigned *h_ctl = nullptr;
     HIPCHK(hipHostMalloc((void **)&h_ctl, 64 * 4, hipHostMallocDefault));
     int rc = SPZ_OK;
@@ -548,13 +559,23 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     for (;;) {
         hipError_t e = hipSuccess;
         for (int k = 0; k < per_poll && e == hipSuccess; ++k) {
-            e = launch_round(w, heur, p, ctx->geo_d.as<PdbGeo>(), ctx->wg, s);
+            if (timing) e = hipEventRecord(ev[2 * k], s);
+            if (e == hipSuccess) e = launch_round(w, heur, p, ctx->geo_d.as<PdbGeo>(), ctx->wg, s);
+            if (timing && e == hipSuccess) e = hipEventRecord(ev[2 * k + 1], s);
             if (e == hipSuccess) e = launch_restart(p, s);
             if (e == hipSuccess) e = launch_control(p, s);
         }
         rounds += per_poll;
         if (e == hipSuccess) e = hipMemcpyAsync(h_ctl, ctx->ctl.p, 64 * 4, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (timing && e == hipSuccess) {
+            for (int k = 0; k < per_poll && e == hipSuccess; ++k) {
+                float ms = 0;
+                e = hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
+                round_ms += ms;
+                ++timed;
+            }
+        }
         if (e != hipSuccess) {
             rc = fail(ctx, SPZ_E_HIP, "search rounds: %s", hipGetErrorString(e));
             break;
@@ -577,10 +598,14 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
             break;
         }
     }
+    if (timing)
+        for (auto &x : ev) hipEventDestroy(x);
     if (rc != SPZ_OK) {
         hipHostFree(h_ctl);
         return rc;
     }
+    ctx->stats[8] = round_ms;
+    ctx->stats[9] = (double)timed;
     ctx->stats[0] = (double)h_ctl[6];
     ctx->stats[2] = (double)h_ctl[7];
     ctx->stats[3] = (double)ctx->n_lanes;
@@ -594,6 +619,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     HIPCHK(hipMemcpyAsync(ic.data(), ctx->iter_counts.p, ic.size() * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     ctx->stats[1] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    double all_nodes = 0;
     for (int i = 0; i < n; ++i) {
         spz_result &r = out[i];
         r.opt_len = opt[i];
@@ -603,9 +629,12 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         uint64_t tot = 0;
         for (int k = 0; k < r.n_iters; ++k) tot += ic[(size_t)i * max_iters + k];
         r.total_nodes = tot;
+        all_nodes += (double)tot;
         if (iter_counts)
             for (int k = 0; k < max_iters; ++k) iter_counts[(size_t)i * max_iters + k] = ic[(size_t)i * max_iters + k];
     }
+    ctx->stats[11] = (double)n;
+    ctx->stats[12] = all_nodes;
     return SPZ_OK;
 }
 

@@ -245,14 +245,15 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                                  (br < W - 1 ? 8u : 0u);
                 uint32_t gen = legal & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
                 cnt += (uint32_t)__builtin_popcount(gen);
-                int slack = L - g - h;
                 uint32_t pass = gen;
-                if (slack < 2) {
-                    // only children with h-1 survive: h changes by exactly +-1
-                    // per move and f keeps the parity of the threshold
-                    uint32_t imp = 0;
-                    const int bl = br * W + bc;
-                    if constexpr (HEUR == 0) {
+                const int bl = br * W + bc;
+                if constexpr (HEUR == 0) {
+                    // Manhattan is consistent: h changes by exactly +-1 per
+                    // move and f keeps the threshold's parity, so with slack
+                    // >= 2 every child fits and with slack 0 only children
+                    // with h-1 do.
+                    if (L - g - h < 2) {
+                        uint32_t imp = 0;
                         int tU = s.bd.tile(br > 0 ? bl - W : bl);
                         int tD = s.bd.tile(br < W - 1 ? bl + W : bl);
                         int tL = s.bd.tile(bc > 0 ? bl - 1 : bl);
@@ -261,18 +262,23 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                         imp |= (tcol<W>(tR) <= bc) ? 2u : 0u;
                         imp |= (tcol<W>(tL) >= bc) ? 4u : 0u;
                         imp |= (trow<W>(tD) <= br) ? 8u : 0u;
-                    } else {
+                        pass = gen & imp;
+                    }
+                } else {
+                    // the PDB max is not consistent (lookups take the minimum
+                    // over blank cells): h may jump by 3, so every generated
+                    // child gets its own f-test; the gathers of all children
+                    // are independent and issue together
+                    pass = 0;
 #pragma unroll
-                        for (int d = 0; d < 4; ++d) {
-                            if (gen & (1u << d)) {
-                                int from = bl + (d == DIR_UP ? -W : (d == DIR_DOWN ? W : (d == DIR_LEFT ? -1 : 1)));
-                                int t = s.bd.tile(from);
-                                int hc = pdb_child_h(P, s.hs, t, from, bl);
-                                imp |= (hc < h) ? (1u << d) : 0u;
-                            }
+                    for (int d = 0; d < 4; ++d) {
+                        if (gen & (1u << d)) {
+                            int from = bl + (d == DIR_UP ? -W : (d == DIR_DOWN ? W : (d == DIR_LEFT ? -1 : 1)));
+                            int t = s.bd.tile(from);
+                            int hc = pdb_child_h(P, s.hs, t, from, bl);
+                            pass |= (g + 1 + hc <= L) ? (1u << d) : 0u;
                         }
                     }
-                    pass = gen & imp;
                 }
                 if (pass) {
                     if (sp < MAXD) {
@@ -390,6 +396,12 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
     }
 }
 
+// largest f on a frontier node's root path (its own f included)
+__device__ __forceinline__ int path_f(const Item &it) {
+    int f = meta_g(it.meta) + meta_h(it.meta);
+    return (int)it.x0 > f ? (int)it.x0 : f;
+}
+
 // ---------------------------------------------------------------------------
 // k_restart: start the current iteration of every instance on the restart
 // list: add the top-tree count from the f-histogram, note goals in the top
@@ -420,10 +432,8 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
         atomicAdd(&s_cnt, c);
         // frontier nodes that fit
         unsigned mine = 0;
-        for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) {
-            uint32_t m = F[i].meta;
-            if (meta_g(m) + meta_h(m) <= L) ++mine;
-        }
+        for (unsigned i = threadIdx.x; i < nf; i += blockDim.x)
+            if (path_f(F[i]) <= L) ++mine;
         atomicAdd(&s_n, mine);
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -436,7 +446,7 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
         __syncthreads();
         for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) {
             Item it = F[i];
-            if (meta_g(it.meta) + meta_h(it.meta) <= L) {
+            if (path_f(it) <= L) {
                 unsigned k = atomicAdd(&s_n, 1u);
                 p.q[(s_base + k) & p.qmask] = it;
             }
@@ -538,8 +548,12 @@ __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *ge
     pos_from_item<W, HEUR>(s, it.b, it.meta, P);
     int g = meta_g(it.meta), prev = meta_prev(it.meta);
     const int br = s.br, bc = s.bc;
+    // F = largest f on the path from the root (x0 carries the parent's F):
+    // the node is expanded in iteration L iff F <= L, heuristic consistent
+    // or not
     int f = g + s.hs.h;
-    if (s.hs.h == 0) atomicMin(&top_goal[inst], g);
+    if ((int)it.x0 > f) f = (int)it.x0;
+    if (s.hs.h == 0) atomicMin(&top_goal[inst], f);
     uint32_t legal = (br > 0 ? 1u : 0u) | (bc < W - 1 ? 2u : 0u) | (bc > 0 ? 4u : 0u) | (br < W - 1 ? 8u : 0u);
     uint32_t gen = legal & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
     atomicAdd(&topgen[(size_t)inst * kFBins + (f < kFBins ? f : kFBins - 1)], (unsigned)__builtin_popcount(gen));
@@ -553,7 +567,7 @@ __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *ge
         c.bd.pack(ci.b);
         ci.meta = meta_pack(c.br * W + c.bc, g + 1, c.hs.h, d);
         ci.inst = (uint32_t)inst;
-        ci.x0 = 0;
+        ci.x0 = (uint32_t)f;
         ci.x1 = 0;
         if (k < (unsigned)p.capl) lvl_out[(size_t)inst * p.capl + k] = ci;
         ++k;

@@ -44,7 +44,7 @@ class Config(ctypes.Structure):
                 ("max_split_depth", ctypes.c_int32), ("workgroup_size", ctypes.c_int32),
                 ("lanes", ctypes.c_int32), ("round_budget", ctypes.c_int32),
                 ("lb_policy", ctypes.c_int32), ("device", ctypes.c_int32),
-                ("queue_items", ctypes.c_int64)]
+                ("time_kernels", ctypes.c_int32), ("queue_items", ctypes.c_int64)]
 
 
 class _Result(ctypes.Structure):
@@ -155,7 +155,8 @@ class Solver:
     def __init__(self, width: int = 4, heuristic: int = H_MANHATTAN, mode: int = MODE_ALL,
                  frontier_target: int = 2048, max_split_depth: int = 24,
                  workgroup_size: int = 256, lanes: int = 0, round_budget: int = 0,
-                 lb_policy: int = LB_UNIFORM, device: int = 0, queue_items: int = 0):
+                 lb_policy: int = LB_UNIFORM, device: int = 0, queue_items: int = 0,
+                 time_kernels: bool = False):
         L = lib()
         cfg = Config()
         L.spz_default_config(ctypes.byref(cfg), width, heuristic)
@@ -168,6 +169,7 @@ class Solver:
         cfg.lb_policy = lb_policy
         cfg.device = device
         cfg.queue_items = queue_items
+        cfg.time_kernels = 1 if time_kernels else 0
         self.cfg = cfg
         self.width = width
         self._ctx = ctypes.c_void_p()
@@ -243,8 +245,9 @@ class Solver:
         return res
 
     def stats(self) -> dict:
-        s = (ctypes.c_double * 8)()
+        s = (ctypes.c_double * 16)()
         self._check(lib().spz_last_stats(self._ctx, s), "spz_last_stats")
         return {"rounds": s[0], "solve_ms": s[1], "donated": s[2], "lanes": s[3],
                 "pdb_levels0": s[4], "pdb_levels1": s[5], "pdb_build_ms": s[6],
-                "overflow_items": s[7]}
+                "overflow_items": s[7], "round_kernel_ms": s[8], "round_launches": s[9],
+                "split_ms": s[10], "instances": s[11], "nodes": s[12]}
