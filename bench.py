@@ -89,35 +89,69 @@ def solve_step(solver, boards, per_instance: bool, max_iters: int):
     return solver.solve(boards, max_iters=max_iters)
 
 
-def cpu_baseline(width, heur, boards, gpu_nodes_per_inst, solver, budget_s=20.0):
-    """The oracle (a single-threaded C port of the reference CPU solver) on
-    the host, over a bounded sample: whole instances of the workload, in
-    order, until ~budget_s of CPU time across a process pool."""
+def cpu_baseline(width, heur, boards, res, solver, budget_s=20.0):
+    """CPU baseline on the host cores, one core, on a bounded sample of the
+    workload: the workload's instances in increasing order of their node
+    count, until about budget_s of CPU time.  Preferred solver: the
+    reference's own (oracle/_ref: src/24puzzle_expand_with_pdb.cpp for the
+    PDB configs, src_horie/idas_cpu_expand.c for the 15-puzzle), else the
+    oracle's C port.  value = measured nodes/s / the workload's mean nodes
+    per instance."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import multiprocessing as mp
+    import tempfile
 
-    import oracle as O
+    import run_ref as R
 
-    cores = min(16, os.cpu_count() or 1)
-    if heur == spz.H_PDB666:
-        tabs = [solver.get_pdb(0), solver.get_pdb(1)]
-        O.pdb_set_tables(*tabs)
-    hname = "pdb" if heur == spz.H_PDB666 else "manhattan"
-    # 1-core rate on the first instances
-    t0 = time.time()
-    nodes = 0
-    done = 0
-    for b in boards:
-        r, thr, c = O.ida(list(b), hname, "all")
-        nodes += sum(c)
-        done += 1
-        if time.time() - t0 > budget_s:
+    order = sorted(range(len(boards)), key=lambda i: res[i].total_nodes)
+    mean_nodes = sum(r.total_nodes for r in res) / len(res)
+    est_rate = 8e6 if heur == spz.H_PDB666 else 60e6
+    pick, acc = [], 0
+    for i in order:
+        if pick and acc + res[i].total_nodes > est_rate * budget_s:
             break
-    dt = time.time() - t0
-    rate = nodes / dt if dt > 0 else 0.0
-    return {"nodes_per_s_1core": rate, "instances_done": done, "seconds": dt,
-            "cores": 1, "sample_nodes": nodes,
-            "projected_instances_per_s": rate / gpu_nodes_per_inst if gpu_nodes_per_inst else None}
+        pick.append(i)
+        acc += res[i].total_nodes
+    if heur == spz.H_PDB666 and R.have("24puzzle_expand_with_pdb"):
+        with tempfile.TemporaryDirectory() as d:
+            paths = (os.path.join(d, "pat24.1256712.tab"), os.path.join(d, "pat24.34891314.tab"))
+            solver.export_pdb_tab(*paths)
+            times = []
+            out = R.yamamura("pdb_all", [list(map(int, boards[i])) for i in pick], paths,
+                             times=times)
+        for (ln, nodes), i in zip(out, pick):
+            assert (ln, nodes) == (res[i].opt_len, res[i].total_nodes), i
+        secs, nodes, kind, who = sum(times), sum(n for _, n in out), "reference", \
+            "src/24puzzle_expand_with_pdb.cpp (oracle/_ref, g++ -O3)"
+    elif heur == spz.H_MANHATTAN and width == 4 and R.have("idas_cpu_expand"):
+        secs, nodes = 0.0, 0
+        for i in pick:
+            dt, work = R.horie_timed(list(map(int, boards[i])))
+            if work is None:
+                continue
+            secs += dt
+            nodes += work
+        kind, who = "reference", "src_horie/idas_cpu_expand.c (oracle/_ref, gcc -O3)"
+    else:
+        import time
+
+        import oracle as O
+
+        if heur == spz.H_PDB666:
+            O.pdb_set_tables(solver.get_pdb(0), solver.get_pdb(1))
+        t0, nodes = time.time(), 0
+        for i in pick:
+            r, thr, c = O.ida(list(map(int, boards[i])),
+                              "pdb" if heur == spz.H_PDB666 else "manhattan", "all")
+            nodes += sum(c)
+        secs, kind, who = time.time() - t0, "port", "oracle/spz_oracle.c (C port)"
+    rate = nodes / secs if secs > 0 else 0.0
+    return {"value": rate / mean_nodes if mean_nodes else None, "unit": "instances/s",
+            "cores": 1, "kind": kind,
+            "sample": f"{who} on {len(pick)} of the workload's instances (smallest node counts "
+                      f"first), {nodes / 1e6:.1f} M nodes in {secs:.1f} s on 1 core = "
+                      f"{rate / 1e6:.2f} M nodes/s; value = that rate / the workload's mean of "
+                      f"{mean_nodes / 1e6:.1f} M nodes per instance",
+            "nodes_per_s": rate}
 
 
 def main():
@@ -125,7 +159,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default=os.environ.get("SPZ_BENCH_CONFIG", "c2"))
+    ap.add_argument("--config", default=os.environ.get("SPZ_BENCH_CONFIG", "c4"))
     ap.add_argument("--n", type=int, default=None, help="instances (default: the config's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -159,8 +193,6 @@ def main():
         st = solver.stats()
         round_ms += st["round_kernel_ms"]
         launches += int(st["round_launches"])
-        if per_instance:
-            pass
         nodes_total += sum(r.total_nodes for r in res)
         if ref is not None:
             assert [(r.opt_len, r.counts) for r in res] == [(r.opt_len, r.counts) for r in ref]
@@ -216,16 +248,7 @@ def main():
         plain.close()
 
     if not args.no_cpu:
-        cb = cpu_baseline(width, heur, boards, nodes_total / args.steps / n, solver)
-        out["cpu_baseline"] = {
-            "value": cb["projected_instances_per_s"], "unit": "instances/s", "cores": cb["cores"],
-            "kind": "port",
-            "sample": f"oracle (C port of the reference CPU IDA*) on the first "
-                      f"{cb['instances_done']} instances of the workload: "
-                      f"{cb['nodes_per_s_1core'] / 1e6:.1f} M nodes/s on 1 core over "
-                      f"{cb['seconds']:.1f} s; value = that rate / the workload's mean "
-                      f"nodes per instance",
-            "nodes_per_s": cb["nodes_per_s_1core"]}
+        out["cpu_baseline"] = cpu_baseline(width, heur, boards, ref, solver)
     print(json.dumps(out), flush=True)
     solver.close()
 

@@ -74,7 +74,7 @@ def horie_counts_by_threshold(board: Sequence[int], timeout: float = 600) -> Dic
 def yamamura_batch(binary: str, set_name: str, n_slots: int,
                    boards: Sequence[Sequence[int]], pad_board: Sequence[int],
                    pdb_files: Optional[Tuple[str, str]] = None,
-                   timeout: float = 3600) -> List[Tuple[int, Optional[int]]]:
+                   timeout: float = 3600, times: Optional[list] = None) -> List[Tuple[int, Optional[int]]]:
     """Run a Yamamura solver whose main loops over
     ../benchmarks/<set_name>/prob000..prob{n_slots-1}.  `boards` fill the
     first slots, `pad_board` the rest.  Returns [(len, nodes|None)] for the
@@ -99,6 +99,12 @@ def yamamura_batch(binary: str, set_name: str, n_slots: int,
         out = subprocess.run([os.path.join(REF_BIN, binary)], cwd=os.path.join(root, "run"),
                              capture_output=True, text=True, timeout=timeout,
                              check=True).stdout
+        if times is not None:
+            # per-instance seconds the solver writes to ../result/*.csv
+            for name in sorted(os.listdir(os.path.join(root, "result"))):
+                with open(os.path.join(root, "result", name)) as f:
+                    vals = [float(x) for x in f.read().split()]
+                times.extend(vals[:len(boards)])
     res = []
     for line in out.splitlines():
         toks = line.split()
@@ -122,7 +128,22 @@ YAMAMURA = {
 }
 
 
-def yamamura(kind: str, boards, pdb_files=None, timeout: float = 3600):
+def yamamura(kind: str, boards, pdb_files=None, timeout: float = 3600, times=None):
     binary, set_name, slots = YAMAMURA[kind]
     goal = list(range(25))
-    return yamamura_batch(binary, set_name, slots, boards, goal, pdb_files, timeout)
+    return yamamura_batch(binary, set_name, slots, boards, goal, pdb_files, timeout, times)
+
+
+def horie_timed(board: Sequence[int], timeout: float = 600) -> Tuple[float, Optional[int]]:
+    """Wall seconds (process start included, a few ms) and the number of
+    nodes Horie's solver generated: every printed iteration plus the first
+    one, whose count equals the h0+1 line (Manhattan parity).  None when it
+    solved the board in its first iteration (no count printed)."""
+    import time
+
+    t = time.perf_counter()
+    lines = horie_iterations(board, timeout)
+    dt = time.perf_counter() - t
+    if len(lines) < 2:
+        return dt, None
+    return dt, sum(c for _, c in lines[1:]) + lines[1][1]

@@ -298,15 +298,28 @@ void orc_pdb_set_tables(const uint8_t *t0, const uint8_t *t1)
     g_tab[1] = t1;
 }
 
+/* the eight lookup specifications, derived once */
+static int g_spec_ready;
+static int g_spec_tab[8];
+static uint8_t g_spec_map[8][25], g_spec_tiles[8][6];
+
+static void spec_init(void)
+{
+    if (g_spec_ready)
+        return;
+    for (int k = 0; k < 8; ++k)
+        orc_pdb_lookup_spec(k, &g_spec_tab[k], g_spec_map[k], g_spec_tiles[k]);
+    g_spec_ready = 1;
+}
+
 static void pdb_lookups_pos(const uint8_t *pos, int out[8])
 {
+    spec_init();
     for (int k = 0; k < 8; ++k) {
-        int tab;
-        uint8_t map[25], tiles[6], cells[6];
-        orc_pdb_lookup_spec(k, &tab, map, tiles);
+        uint8_t cells[6];
         for (int j = 0; j < 6; ++j)
-            cells[j] = map[pos[tiles[j]]];
-        out[k] = g_tab[tab][rank_k(25, 6, cells)];
+            cells[j] = g_spec_map[k][pos[g_spec_tiles[k][j]]];
+        out[k] = g_tab[g_spec_tab[k]][rank_k(25, 6, cells)];
     }
 }
 
@@ -389,8 +402,7 @@ static int pdb_build_generic(int w, int k, const uint8_t *tiles, int goal_kind,
         return -1;
     }
 #ifdef _OPENMP
-    if (n_threads > 0)
-        omp_set_num_threads(n_threads);
+    omp_set_num_threads(n_threads > 0 ? n_threads : omp_get_num_procs());
 #else
     (void)n_threads;
 #endif

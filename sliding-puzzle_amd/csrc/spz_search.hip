@@ -364,6 +364,11 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
         }
     }
 
+    // keep the per-lane u32 count far from wrapping on long-lived items
+    if (busy && cnt >= (1u << 30)) {
+        atomicAdd(&p.cnt[inst], (unsigned long long)cnt);
+        cnt = 0;
+    }
     uint64_t busy_mask = __ballot(busy);
     if (lane == 0 && busy_mask) atomicAdd(&p.ctl[2], (unsigned)__popcll(busy_mask));
     if (busy) {
@@ -531,14 +536,17 @@ __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *ge
     const int inst = (int)(gid / (size_t)p.capl);
     const unsigned j = (unsigned)(gid % (size_t)p.capl);
     if (inst >= p.n_inst) return;
-    if (frozen[inst]) return;
+    // frozen[inst] = 1 + the depth at which the instance froze; it is written
+    // during that launch, so this launch only honours earlier freezes
+    const int fz = frozen[inst];
+    if (fz != 0 && fz - 1 < depth) return;
     const unsigned n = min(n_in[inst], (unsigned)p.capl);
     const bool freeze = (n >= target) || (depth >= max_depth);
     if (freeze) {
         if (j < n) ((Item *)p.frontier)[(size_t)inst * p.capl + j] = lvl_in[(size_t)inst * p.capl + j];
         if (j == 0) {
             nfront[inst] = n;
-            frozen[inst] = 1;
+            frozen[inst] = depth + 1;
         }
         return;
     }

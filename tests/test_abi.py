@@ -1,0 +1,82 @@
+"""The C ABI without a GPU: libspz.so loads, exports every entry point that
+include/spz.h declares, and its host-only functions (generators, PDB lookup
+geometry, problem-file parsing) agree with the oracle."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import spz
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "spz.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(spz_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = spz.lib()
+    names = declared_functions()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(lib, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", spz.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\b(spz_[a-z0-9_]+)\b", out))
+    assert set(names) <= exported
+    assert set(spz.EXPORTS) == set(names)
+
+
+def test_header_compiles_as_c():
+    src = '#include "spz.h"\nint main(void){spz_config c; spz_default_config(&c,4,0); return c.width!=4;}\n'
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(REPO, "include"),
+                        "-x", "c", "-", "-fsyntax-only"], input=src, text=True, capture_output=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("w", [4, 5])
+def test_generators_match_oracle(w):
+    for i in range(30):
+        assert (spz.gen_walk(w, 40 + i, 1_000_000 + i) == O.walk(w, 40 + i, 1_000_000 + i)).all()
+        assert (spz.gen_random(w, 2_000_000 + i) == O.random_solvable(w, 2_000_000 + i)).all()
+
+
+def test_pdb_geometry_matches_oracle():
+    tiles, maps = spz.pdb_geometry()
+    seen_reg, seen_ref = set(), set()
+    for k in range(8):
+        tab, m, t = O.pdb_lookup_spec(k)
+        assert list(tiles[k]) == t, k
+        assert list(maps[k]) == m, k
+        (seen_reg if k < 4 else seen_ref).update(t)
+    # each partition covers every tile exactly once
+    assert seen_reg == set(range(1, 25)) and seen_ref == set(range(1, 25))
+
+
+def test_read_problem_and_format(tmp_path, ref_sets):
+    p = tmp_path / "prob000"
+    b = ref_sets["korf100"][0]
+    p.write_text(" ".join(map(str, b)) + " \n")
+    assert list(spz.read_problem(str(p))) == b
+    p5 = tmp_path / "001"
+    b5 = ref_sets["korf50_24puzzle"][0]
+    p5.write_text(" ".join(map(str, b5)))  # no trailing newline, like Korf50 files
+    assert list(spz.read_problem(str(p5))) == b5
+    r = spz.Result(opt_len=45, h0=41, n_iters=3, status=0, total_nodes=668555)
+    assert spz.format_line("../benchmarks/korf100/prob000", r) == \
+        "../benchmarks/korf100/prob000 45 668555"
+
+
+def test_default_config_values():
+    import ctypes
+    cfg = spz.Config()
+    spz.lib().spz_default_config(ctypes.byref(cfg), 5, spz.H_PDB666)
+    assert (cfg.width, cfg.heuristic, cfg.mode, cfg.frontier_target) == (5, 1, 0, 2048)
+    assert cfg.workgroup_size == 256 and cfg.time_kernels == 0

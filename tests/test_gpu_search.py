@@ -135,3 +135,18 @@ def test_bad_input_rejected(s15):
         s15.solve([[0, 2, 1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15]])  # tiles 1,2 swapped
     with pytest.raises(spz.SpzError):
         s15.solve([[0] * 16])
+
+
+def test_repeat_solves_same_context(golden):
+    """Back-to-back solves on one context with instances that freeze their
+    root split at different depths: buffers from the previous solve must not
+    leak into the next (regression test for the split-freeze ordering)."""
+    fx = golden("iterations_c1_walk40.json")
+    recs = [r for r in fx["instances"] if "error" not in r]
+    kx = golden("iterations_korf100.json")["instances"][:8]
+    with spz.Solver(width=4, frontier_target=4096) as s:
+        for rep in range(3):
+            say("repeat", rep)
+            res = s.solve([r["board"] for r in kx] + [r["board"] for r in recs])
+            for rec, r in zip(kx + recs, res):
+                check_counts(rec, r)

@@ -25,6 +25,7 @@ def nbr(w, cell, d):
 
 
 def md(w, board):
+    board = [int(x) for x in board]
     return sum(abs(i // w - t // w) + abs(i % w - t % w) for i, t in enumerate(board) if t)
 
 
@@ -88,7 +89,7 @@ def dp_counts(w, board, limit):
     (state, last move) layer by layer; a node is expanded when its f fits the
     limit (f is monotone, so ancestors fit too) and then contributes one
     count per generated child, times its multiplicity."""
-    layer = {(tuple(board), -1): 1}
+    layer = {(tuple(int(x) for x in board), -1): 1}
     total, g, found = 0, 0, False
     while layer:
         nxt = defaultdict(int)
