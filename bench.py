@@ -80,6 +80,18 @@ WORKLOAD = {
 }
 
 
+def traffic_per_node(config: str):
+    """Calibrated HBM-side bytes per generated node of k_round, measured by
+    two rocprofv3 --pmc passes on this workload (profiles/ summary)."""
+    path = os.path.join(REPO, "profiles", f"k_round_{config}_pdb_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_node:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def solve_step(solver, boards, per_instance: bool, max_iters: int):
     if per_instance:
         out = []
@@ -166,17 +178,32 @@ def main():
     ap.add_argument("--max-iters", type=int, default=64)
     ap.add_argument("--round-budget", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--lds-depth", type=int, default=0)
+    ap.add_argument("--wg", type=int, default=256)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run one plain device-to-device copy of MIB MiB (a kernel of "
+                         "known traffic that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this solver runs on one GPU; --gpus above 1 is not supported "
                                    "(instances shard trivially across GPUs, see DESIGN.md)"}))
         sys.exit(2)
 
+    if args.copy_probe:
+        import torch
+
+        a = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
     width, heur, boards = instances(args.config, args.n)
     boards = np.stack(boards)
     per_instance = args.config == "c2"
     solver = spz.Solver(width=width, heuristic=heur, frontier_target=FRONTIER[args.config],
-                        round_budget=args.round_budget, lanes=args.lanes, time_kernels=True)
+                        round_budget=args.round_budget, lanes=args.lanes, time_kernels=True,
+                        lds_stack_depth=args.lds_depth, workgroup_size=args.wg)
     pdb_ms = None
     if heur == spz.H_PDB666:
         t = time.time()
@@ -187,11 +214,15 @@ def main():
     for _ in range(args.warmup):
         ref = solve_step(solver, boards, per_instance, args.max_iters)
     round_ms, launches, nodes_total = 0.0, 0, 0
+    brk = {"solve_ms": 0.0, "setup_ms": 0.0, "split_ms": 0.0, "rounds_wall_ms": 0.0, "rounds": 0.0,
+           "donated": 0.0}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         res = solve_step(solver, boards, per_instance, args.max_iters)
         st = solver.stats()
         round_ms += st["round_kernel_ms"]
+        for k in brk:
+            brk[k] += st[k] / args.steps
         launches += int(st["round_launches"])
         nodes_total += sum(r.total_nodes for r in res)
         if ref is not None:
@@ -217,6 +248,11 @@ def main():
         achieved = nodes_per_launch * 16 / avg_launch_s / 1e12
         roof = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
                 "frac": achieved / VALU_PEAK_TOPS, "traffic": None}
+    tb = traffic_per_node(args.config)
+    if tb is not None:
+        roof["traffic"] = tb * nodes_per_launch
+        roof["traffic_source"] = "calibrated FETCH_SIZE+WRITE_SIZE bytes per generated node from " \
+                                 "profiles/k_round_c4_pdb_traffic.md x this run's nodes per launch"
     roof["kernel"] = "k_round"
     roof["avg_launch_ms"] = avg_launch_s * 1e3
     roof["launches"] = launches
@@ -230,9 +266,16 @@ def main():
                    "frontier_target": FRONTIER[args.config], "parallelism": "single GPU"},
         "nodes_per_s": nodes_per_s, "nodes_per_step": nodes_total / args.steps,
         "unsolved_within_max_iters": unsolved, "roofline": roof,
+        "breakdown_per_step": dict(brk, round_kernel_ms=round_ms / args.steps,
+                                   lanes=solver.stats()["lanes"]),
     }
     if pdb_ms is not None:
         out["pdb_build_ms"] = pdb_ms
+        _, gps = solver.gather_probe(1 << 30)
+        roof["measured_gather_peak"] = {
+            "gathers_per_s": gps, "GB_per_s_at_64B": gps * 64 / 1e9,
+            "frac_of_it": achieved / (gps * 64 / 1e9),
+            "what": "uniform random u8 gathers over the two 244 MB direct tables (spz_gather_probe)"}
 
     if not args.no_plain and not per_instance:
         # plain GPU IDA*: one instance per launch, uniform static split

@@ -68,6 +68,9 @@ typedef struct spz_config {
     int32_t workgroup_size;  /* 64, 128 or 256 lanes (0 = 256) */
     int32_t lanes;           /* DFS lanes in flight (0 = fill the device) */
     int32_t round_budget;    /* DFS steps per lane per round (0 = auto) */
+    int32_t lds_stack_depth; /* DFS stack levels per lane kept in LDS (0 =
+                                80 for width 4, 96 for width 5; at most 160);
+                                deeper subtrees spill to the work queue */
     int32_t lb_policy;       /* SPZ_LB_* */
     int32_t device;          /* HIP device ordinal */
     int32_t time_kernels;    /* 1: time every search-round launch with HIP
@@ -129,11 +132,17 @@ int spz_set_lb(spz_ctx *ctx, int32_t lb_policy);
  *  3 lanes                 4,5 PDB BFS levels     6 PDB build ms
  *  7 stack-overflow items  8 round-kernel ms (sum, when time_kernels)
  *  9 round launches timed  10 split ms            11 instances
- *  12 nodes (all instances, all iterations) */
+ *  12 nodes (all instances, all iterations)
+ *  13 setup ms (upload, root split, first restart)  14 rounds wall ms */
 int spz_last_stats(spz_ctx *ctx, double *stats);
 
 const char *spz_last_error(spz_ctx *ctx);
 void spz_destroy(spz_ctx *ctx);
+
+/* Diagnostics: uniformly random byte gathers over the two direct tables
+ * (the search's access pattern without the search) -- the measured
+ * random-access ceiling for the PDB kernel.  No reference counterpart. */
+int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, double *ms, double *gathers_per_s);
 
 /* Diagnostics: the eight PDB lookups as (tiles in slot order, cell map),
  * derived from the cell symmetries; the reference writes them out as

@@ -14,7 +14,7 @@
 
 namespace spz {
 hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo, int wg, hipStream_t s);
-size_t round_lds_bytes(int width, int wg);
+size_t round_lds_bytes(int maxd, int wg);
 const void *round_kernel_ptr(int width, int heur);
 hipError_t launch_restart(const KParams &p, hipStream_t s);
 hipError_t launch_control(const KParams &p, hipStream_t s);
@@ -28,6 +28,8 @@ hipError_t launch_scatter_roots(const Item *roots, Item *lvl, int capl, int n, h
 hipError_t pdb_bfs_build(int pat, uint8_t *compact, uint32_t *work, unsigned long long *d_cnt, hipStream_t s,
                          int *levels_out);
 hipError_t pdb_spread(const uint8_t *compact, uint8_t *direct, hipStream_t s);
+hipError_t launch_gather_probe(const uint8_t *t0, const uint8_t *t1, int blocks, int per_thread, uint64_t seed,
+                               uint32_t *out, hipStream_t s);
 constexpr uint64_t kPlacementsHost = 127512000ull;
 } // namespace spz
 
@@ -146,6 +148,7 @@ void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic) {
     cfg->workgroup_size = 256;
     cfg->lanes = 0;
     cfg->round_budget = 0;
+    cfg->lds_stack_depth = 0;
     cfg->lb_policy = SPZ_LB_UNIFORM;
     cfg->device = 0;
     cfg->time_kernels = 0;
@@ -170,6 +173,11 @@ int spz_create(spz_ctx **out, const spz_config *cfg) {
     }
     if (ctx->cfg.frontier_target < 1) ctx->cfg.frontier_target = 1;
     if (ctx->cfg.max_split_depth <= 0) ctx->cfg.max_split_depth = 24;
+    if (ctx->cfg.lds_stack_depth <= 0) ctx->cfg.lds_stack_depth = cfg->width == 4 ? Geo<4>::DEFD : Geo<5>::DEFD;
+    if (ctx->cfg.lds_stack_depth < 8 || ctx->cfg.lds_stack_depth > Geo<5>::MAXD) {
+        delete ctx;
+        return SPZ_E_ARG;
+    }
     int rc = SPZ_OK;
     do {
         hipError_t e = hipSetDevice(cfg->device);
@@ -193,7 +201,8 @@ int spz_create(spz_ctx **out, const spz_config *cfg) {
         if (lanes <= 0) {
             int blocks = 0;
             e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &blocks, round_kernel_ptr(cfg->width, cfg->heuristic), wg, round_lds_bytes(cfg->width, wg));
+                &blocks, round_kernel_ptr(cfg->width, cfg->heuristic), wg,
+                round_lds_bytes(ctx->cfg.lds_stack_depth, wg));
             if (e != hipSuccess || blocks <= 0) blocks = 2;
             lanes = ctx->n_cu * blocks * wg;
         }
@@ -206,7 +215,7 @@ int spz_create(spz_ctx **out, const spz_config *cfg) {
             break;
         }
         size_t L = (size_t)lanes;
-        int maxd = cfg->width == 4 ? Geo<4>::MAXD : Geo<5>::MAXD;
+        int maxd = ctx->cfg.lds_stack_depth;
         if ((e = ctx->c_st.ensure(L * 4)) != hipSuccess || (e = ctx->c_rows.ensure(L * 4 * 5)) != hipSuccess ||
             (e = ctx->c_root.ensure(L * 4 * 4)) != hipSuccess || (e = ctx->c_meta.ensure(L * 4)) != hipSuccess ||
             (e = ctx->c_rmeta.ensure(L * 4)) != hipSuccess || (e = ctx->c_inst.ensure(L * 4)) != hipSuccess ||
@@ -378,6 +387,7 @@ static KParams base_params(spz_ctx *ctx, int n, int max_iters, int capl) {
     p.top_goal = ctx->top_goal.as<int>();
     p.capl = capl;
     p.n_lanes = ctx->n_lanes;
+    p.maxd = ctx->cfg.lds_stack_depth;
     p.c_st = ctx->c_st.as<uint32_t>();
     p.c_rows = ctx->c_rows.as<uint32_t>();
     p.c_root = ctx->c_root.as<uint32_t>();
@@ -539,6 +549,8 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
 
     HIPCHK(launch_restart(p, s));
     HIPCHK(launch_control(p, s));
+    auto t_rounds = std::chrono::steady_clock::now();
+    ctx->stats[13] = std::chrono::duration<double, std::milli>(t_rounds - t_start).count();
 
     // rounds until every instance is done; poll every few rounds.  Each
     // round is a bounded kernel; the host stops on a wall-clock limit or if
@@ -604,6 +616,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         hipHostFree(h_ctl);
         return rc;
     }
+    ctx->stats[14] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_rounds).count();
     ctx->stats[8] = round_ms;
     ctx->stats[9] = (double)timed;
     ctx->stats[0] = (double)h_ctl[6];
@@ -635,6 +648,37 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     }
     ctx->stats[11] = (double)n;
     ctx->stats[12] = all_nodes;
+    return SPZ_OK;
+}
+
+int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, double *ms, double *gathers_per_s) {
+    if (!ctx || !ms || !gathers_per_s || n_gathers <= 0) return SPZ_E_ARG;
+    if (!ctx->have_pdb) return fail(ctx, SPZ_E_NOPDB, "call spz_build_pdb first");
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    const int per_thread = 256;
+    long long threads = (n_gathers + per_thread - 1) / per_thread;
+    int blocks = (int)std::max<long long>(1, (threads + 255) / 256);
+    DBuf out;
+    HIPCHK(out.ensure((size_t)blocks * 256 * 4));
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    hipError_t e = launch_gather_probe(ctx->pdb_direct[0].as<uint8_t>(), ctx->pdb_direct[1].as<uint8_t>(), blocks,
+                                       per_thread, 12345, out.as<uint32_t>(), ctx->stream); // warm
+    if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
+    if (e == hipSuccess)
+        e = launch_gather_probe(ctx->pdb_direct[0].as<uint8_t>(), ctx->pdb_direct[1].as<uint8_t>(), blocks,
+                                per_thread, 777, out.as<uint32_t>(), ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    out.release();
+    if (e != hipSuccess) return fail(ctx, SPZ_E_HIP, "gather probe: %s", hipGetErrorString(e));
+    *ms = t;
+    *gathers_per_s = (double)blocks * 256.0 * per_thread / (t * 1e-3);
     return SPZ_OK;
 }
 

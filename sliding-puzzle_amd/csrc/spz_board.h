@@ -13,13 +13,19 @@
 namespace spz {
 
 template <int W> struct Geo;
+// MAXD: upper bound of the per-lane LDS stack depth (the depth in use is a
+// launch parameter); DEFD: default depth.  A lane whose stack is full turns
+// the children it would push into queue items, so depth never limits
+// correctness.
 template <> struct Geo<4> {
     static constexpr int N2 = 16;
-    static constexpr int MAXD = 96; // LDS stack levels per lane
+    static constexpr int MAXD = 160;
+    static constexpr int DEFD = 80;
 };
 template <> struct Geo<5> {
     static constexpr int N2 = 25;
     static constexpr int MAXD = 160;
+    static constexpr int DEFD = 96;
 };
 
 // goal row / column of tile t (tile t belongs at cell t)
@@ -164,8 +170,10 @@ __device__ __forceinline__ int pdb_read_k(const PdbCtx &P, int k03, uint32_t idx
 }
 
 // Heuristic of the child reached by moving tile t from cell a to cell b,
-// without changing the state (f-tests at expansion).
-__device__ __forceinline__ int pdb_child_h(const PdbCtx &P, const HState<1> &s, int t, int a, int b) {
+// without changing the state (f-tests at expansion); the two values read
+// are returned packed (regular | reflected << 8) for reuse by pdb_apply_known.
+__device__ __forceinline__ int pdb_child_h(const PdbCtx &P, const HState<1> &s, int t, int a, int b,
+                                           uint32_t &vals) {
     const PdbGeo &G = *P.geo;
     int kr = G.reg_k[t], kf = G.ref_k[t] - 4;
     uint32_t ir = sel4(s.i0, s.i1, s.i2, s.i3, kr) +
@@ -174,9 +182,27 @@ __device__ __forceinline__ int pdb_child_h(const PdbCtx &P, const HState<1> &s, 
                    (uint32_t)((int)G.map[kf + 4][b] - (int)G.map[kf + 4][a]) * G.ref_mul[t];
     int vr = pdb_read_k(P, kr, ir);
     int vf = pdb_read_k(P, kf, iff);
+    vals = (uint32_t)vr | ((uint32_t)vf << 8);
     int A = s.A - byte_of(s.v0, kr) + vr;
     int B = s.B - byte_of(s.v1, kf) + vf;
     return A > B ? A : B;
+}
+
+// pdb_apply with the two new values already known (no gathers)
+__device__ __forceinline__ void pdb_apply_known(const PdbCtx &P, HState<1> &s, int t, int a, int b,
+                                                uint32_t vals) {
+    const PdbGeo &G = *P.geo;
+    int kr = G.reg_k[t], kf = G.ref_k[t] - 4;
+    uint32_t dr = (uint32_t)((int)G.map[kr][b] - (int)G.map[kr][a]) * G.reg_mul[t];
+    uint32_t df = (uint32_t)((int)G.map[kf + 4][b] - (int)G.map[kf + 4][a]) * G.ref_mul[t];
+    add4(s.i0, s.i1, s.i2, s.i3, kr, dr);
+    add4(s.j0, s.j1, s.j2, s.j3, kf, df);
+    int vr = (int)(vals & 0xffu), vf = (int)((vals >> 8) & 0xffu);
+    s.A += vr - byte_of(s.v0, kr);
+    s.B += vf - byte_of(s.v1, kf);
+    s.v0 = set_byte(s.v0, kr, vr);
+    s.v1 = set_byte(s.v1, kf, vf);
+    s.h = s.A > s.B ? s.A : s.B;
 }
 
 __device__ __forceinline__ void pdb_apply(const PdbCtx &P, HState<1> &s, int t, int a, int b) {
@@ -243,6 +269,19 @@ __device__ __forceinline__ void do_move(Pos<W, HEUR> &s, int d, const PdbCtx &P)
     } else {
         pdb_apply(P, s.hs, t, from, to);
     }
+    s.bd.slide(t, from, to);
+    s.br = nr;
+    s.bc = nc;
+}
+
+// do_move for the PDB state with the moved tile's two lookup values known
+template <int W>
+__device__ __forceinline__ void do_move_known(Pos<W, 1> &s, int d, const PdbCtx &P, uint32_t vals) {
+    int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
+    int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
+    int from = nr * W + nc, to = s.br * W + s.bc;
+    int t = s.bd.tile(from);
+    pdb_apply_known(P, s.hs, t, from, to, vals);
     s.bd.slide(t, from, to);
     s.br = nr;
     s.bc = nc;

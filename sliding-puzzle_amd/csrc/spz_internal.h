@@ -72,6 +72,7 @@ struct KParams {
 
     // lane contexts, structure of arrays over n_lanes lanes
     int n_lanes;
+    int maxd;          // LDS stack levels per lane in use
     uint32_t *c_st;    // state | sp<<8
     uint32_t *c_rows;  // [5][n_lanes]
     uint32_t *c_root;  // [4][n_lanes] packed item root

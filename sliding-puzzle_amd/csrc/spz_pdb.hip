@@ -212,6 +212,34 @@ __global__ void __launch_bounds__(256) k_pdb_spread(const uint8_t *__restrict__ 
     direct[i] = ok ? compact[pl_rank(s)] : (uint8_t)0;
 }
 
+// Random-gather probe: every thread reads `per_thread` bytes at uniformly
+// random positions of the two direct tables (independent addresses, so many
+// are in flight), the access pattern of the PDB search without its search.
+__global__ void __launch_bounds__(256) k_gather_probe(const uint8_t *__restrict__ t0, const uint8_t *__restrict__ t1,
+                                                      int per_thread, uint64_t seed, uint32_t *out) {
+    uint64_t x = seed ^ ((uint64_t)(blockIdx.x * blockDim.x + threadIdx.x) * 0x9e3779b97f4a7c15ull);
+    uint32_t acc = 0;
+    for (int k = 0; k < per_thread; k += 4) {
+        uint32_t i[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            x ^= x >> 12;
+            x ^= x << 25;
+            x ^= x >> 27;
+            uint64_t r = x * 0x2545f4914f6cdd1dull;
+            i[j] = (uint32_t)(((r >> 32) * kPdbDirect) >> 32);
+        }
+        acc += t0[i[0]] + t1[i[1]] + t0[i[2]] + t1[i[3]];
+    }
+    out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+hipError_t launch_gather_probe(const uint8_t *t0, const uint8_t *t1, int blocks, int per_thread, uint64_t seed,
+                               uint32_t *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_probe, dim3(blocks), dim3(256), 0, s, t0, t1, per_thread, seed, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Host driver: builds canonical table `pat` (0 or 1) into `compact` (device,
 // kPlacements bytes).  Scratch: 3 x kPlacements u32.  Returns levels run.

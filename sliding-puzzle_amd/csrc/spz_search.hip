@@ -90,7 +90,7 @@ __device__ __forceinline__ void pos_from_item(Pos<W, HEUR> &s, const uint32_t (&
 // try there (4-bit mask of children that passed the f-test).
 template <int W, int HEUR>
 __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
-    constexpr int MAXD = Geo<W>::MAXD;
+    const int MAXD = p.maxd;
     extern __shared__ uint8_t lds[];
     __shared__ PdbGeo geo_s;
     const int lane = threadIdx.x & 63;
@@ -247,6 +247,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                 cnt += (uint32_t)__builtin_popcount(gen);
                 uint32_t pass = gen;
                 const int bl = br * W + bc;
+                uint32_t cvals = 0, cvals2 = 0; // PDB values of children 0,1 / 2,3 (16 bits each)
                 if constexpr (HEUR == 0) {
                     // Manhattan is consistent: h changes by exactly +-1 per
                     // move and f keeps the threshold's parity, so with slack
@@ -275,8 +276,13 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                         if (gen & (1u << d)) {
                             int from = bl + (d == DIR_UP ? -W : (d == DIR_DOWN ? W : (d == DIR_LEFT ? -1 : 1)));
                             int t = s.bd.tile(from);
-                            int hc = pdb_child_h(P, s.hs, t, from, bl);
+                            uint32_t v;
+                            int hc = pdb_child_h(P, s.hs, t, from, bl, v);
                             pass |= (g + 1 + hc <= L) ? (1u << d) : 0u;
+                            if (d < 2)
+                                cvals |= v << (16 * d);
+                            else
+                                cvals2 |= v << (16 * (d - 2));
                         }
                     }
                 }
@@ -286,7 +292,12 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                         uint32_t R = pass & (pass - 1);
                         stk[sp * sstride] = (uint8_t)((R << 2) | (uint32_t)d);
                         ++sp;
-                        do_move<W, HEUR>(s, d, P);
+                        if constexpr (HEUR == 1) {
+                            uint32_t w2 = (d >= 2) ? cvals2 : cvals;
+                            do_move_known<W>(s, d, P, (w2 >> (16 * (d & 1))) & 0xffffu);
+                        } else {
+                            do_move<W, HEUR>(s, d, P);
+                        }
                         ++g;
                         prev = d;
                     } else {
@@ -623,7 +634,7 @@ __global__ void k_scatter_roots(const Item *roots, Item *lvl, int capl, int n) {
 // host-side launchers
 template <int W, int HEUR>
 static hipError_t launch_round_t(const KParams &p, const PdbGeo *geo, int wg, hipStream_t s) {
-    size_t lds = (size_t)Geo<W>::MAXD * wg;
+    size_t lds = (size_t)p.maxd * wg;
     hipLaunchKernelGGL((k_round<W, HEUR>), dim3(p.n_lanes / wg), dim3(wg), lds, s, p, geo);
     return hipGetLastError();
 }
@@ -634,7 +645,7 @@ hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo
     return launch_round_t<5, 1>(p, geo, wg, s);
 }
 
-size_t round_lds_bytes(int width, int wg) { return (size_t)(width == 4 ? Geo<4>::MAXD : Geo<5>::MAXD) * wg; }
+size_t round_lds_bytes(int maxd, int wg) { return (size_t)maxd * wg; }
 
 const void *round_kernel_ptr(int width, int heur) {
     if (width == 4) return (const void *)&k_round<4, 0>;

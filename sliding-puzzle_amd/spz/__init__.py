@@ -43,6 +43,7 @@ class Config(ctypes.Structure):
                 ("mode", ctypes.c_int32), ("frontier_target", ctypes.c_int32),
                 ("max_split_depth", ctypes.c_int32), ("workgroup_size", ctypes.c_int32),
                 ("lanes", ctypes.c_int32), ("round_budget", ctypes.c_int32),
+                ("lds_stack_depth", ctypes.c_int32),
                 ("lb_policy", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("time_kernels", ctypes.c_int32), ("queue_items", ctypes.c_int64)]
 
@@ -70,7 +71,7 @@ _lib = None
 EXPORTS = ["spz_default_config", "spz_create", "spz_build_pdb", "spz_export_pdb_tab",
            "spz_get_pdb", "spz_solve_batch", "spz_eval_h", "spz_set_lb", "spz_last_stats",
            "spz_last_error", "spz_destroy", "spz_pdb_geometry", "spz_gen_walk",
-           "spz_gen_random"]
+           "spz_gen_random", "spz_gather_probe"]
 
 
 def lib() -> ctypes.CDLL:
@@ -97,6 +98,8 @@ def lib() -> ctypes.CDLL:
     L.spz_last_error.restype = ctypes.c_char_p
     L.spz_destroy.argtypes = [vp]
     L.spz_destroy.restype = None
+    L.spz_gather_probe.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_double)]
     L.spz_pdb_geometry.argtypes = [u8p, u8p]
     L.spz_pdb_geometry.restype = None
     L.spz_gen_walk.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, u8p]
@@ -156,7 +159,7 @@ class Solver:
                  frontier_target: int = 2048, max_split_depth: int = 24,
                  workgroup_size: int = 256, lanes: int = 0, round_budget: int = 0,
                  lb_policy: int = LB_UNIFORM, device: int = 0, queue_items: int = 0,
-                 time_kernels: bool = False):
+                 time_kernels: bool = False, lds_stack_depth: int = 0):
         L = lib()
         cfg = Config()
         L.spz_default_config(ctypes.byref(cfg), width, heuristic)
@@ -170,6 +173,7 @@ class Solver:
         cfg.device = device
         cfg.queue_items = queue_items
         cfg.time_kernels = 1 if time_kernels else 0
+        cfg.lds_stack_depth = lds_stack_depth
         self.cfg = cfg
         self.width = width
         self._ctx = ctypes.c_void_p()
@@ -216,6 +220,12 @@ class Solver:
         self._check(lib().spz_get_pdb(self._ctx, pattern, _u8p(out)), "spz_get_pdb")
         return out
 
+    def gather_probe(self, n_gathers: int = 1 << 30):
+        ms, gps = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().spz_gather_probe(self._ctx, n_gathers, ctypes.byref(ms), ctypes.byref(gps)),
+                    "spz_gather_probe")
+        return ms.value, gps.value
+
     def set_lb(self, policy: int):
         self._check(lib().spz_set_lb(self._ctx, policy), "spz_set_lb")
 
@@ -250,4 +260,5 @@ class Solver:
         return {"rounds": s[0], "solve_ms": s[1], "donated": s[2], "lanes": s[3],
                 "pdb_levels0": s[4], "pdb_levels1": s[5], "pdb_build_ms": s[6],
                 "overflow_items": s[7], "round_kernel_ms": s[8], "round_launches": s[9],
-                "split_ms": s[10], "instances": s[11], "nodes": s[12]}
+                "split_ms": s[10], "instances": s[11], "nodes": s[12],
+                "setup_ms": s[13], "rounds_wall_ms": s[14]}
