@@ -47,7 +47,10 @@ extern "C" {
 /* dispatch order of frontier nodes (the reference's static LB is
  * src/15puzzle_block_parallel.cu:490-563) */
 #define SPZ_LB_UNIFORM 0 /* frontier order */
-#define SPZ_LB_LEARNED 1 /* longest predicted subtree first (needs weights) */
+#define SPZ_LB_LEARNED 1 /* longest predicted subtree first (MLP, needs parameters) */
+#define SPZ_LB_PREVLOAD 2 /* longest subtree of the previous iteration first (the
+                            reference's load-based assignment,
+                            src_horie/idas_dynamic.cu:1527-1614) */
 
 /* per-instance status */
 #define SPZ_ST_SOLVED 0
@@ -75,6 +78,9 @@ typedef struct spz_config {
     int32_t device;          /* HIP device ordinal */
     int32_t time_kernels;    /* 1: time every search-round launch with HIP
                                 events on the solver's stream (stats 8-9) */
+    int32_t log_samples;     /* >0: keep up to this many (frontier node,
+                                threshold, subtree count) samples per solve
+                                (spz_get_samples); training data of the MLP */
     int64_t queue_items;     /* work-queue capacity in items (0 = auto) */
 } spz_config;
 
@@ -123,9 +129,35 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n,
  * computed on the device; h must hold n int32. */
 int spz_eval_h(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h);
 
-/* Change the dispatch policy / learned-LB weights between solves.
- * weights: bf16 MLP parameters in the layout of spz_mlp_param_count(). */
+/* Change the dispatch policy (SPZ_LB_*) between solves.  The policy only
+ * orders the frontier nodes an iteration queues; results never depend on
+ * it.  Replaces the host re-split of heavy roots between iterations
+ * (src/15puzzle_block_parallel.cu:490-563). */
 int spz_set_lb(spz_ctx *ctx, int32_t lb_policy);
+
+/* Learned load balancing (no reference counterpart): an MLP
+ * 640 -> 256 -> 256 -> 1 (ReLU, bf16 weights/activations, fp32 biases and
+ * accumulation) predicting log2(1 + subtree size) of a frontier node at a
+ * threshold.  Features: one-hot of tile t at cell c (index t*width^2 + c),
+ * h/32 at 625, (threshold - h)/32 at 626.  Parameters are fp32, flat:
+ * W1[640][256], b1[256], W2[256][256], b2[256], w3[256], b3. */
+int32_t spz_mlp_param_count(void);
+int spz_mlp_init(spz_ctx *ctx, uint64_t seed);
+int spz_mlp_set_params(spz_ctx *ctx, const float *params);
+int spz_mlp_get_params(spz_ctx *ctx, float *params);
+/* forward pass on the GPU: out[i] for boards[i] at threshold thr[i] */
+int spz_mlp_forward(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, int32_t n, float *out);
+/* loss = mean (y - target)^2 and its gradient w.r.t. the parameters */
+int spz_mlp_grad(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const float *target, int32_t n,
+                 float *loss, float *grad);
+/* Adam training on the GPU; epoch_loss (may be NULL) gets the mean batch
+ * loss of each epoch */
+int spz_mlp_train(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const float *target, int32_t n,
+                  int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss);
+/* Samples logged by the last solve (config log_samples): boards (n x
+ * width^2), thresholds and subtree counts.  *n_out = number copied; with
+ * boards == NULL, the number available. */
+int spz_get_samples(spz_ctx *ctx, uint8_t *boards, int32_t *thr, uint64_t *loads, int32_t max, int32_t *n_out);
 
 /* Statistics of the last solve (stats must hold 16 doubles):
  *  0 rounds run            1 solve wall ms        2 items donated

@@ -1,6 +1,7 @@
 // spz_api.hip -- host implementation of the libspz C ABI (include/spz.h).
 #include "../../include/spz.h"
 #include "spz_board.h"
+#include "spz_mlp.h"
 
 #include <stdarg.h>
 #include <stdio.h>
@@ -84,6 +85,10 @@ struct spz_ctx {
     // per instance
     DBuf roots, L, it, cnt, pend, found, done, opt, iter_counts, restart_list;
     DBuf frontier, nfront, topgen, top_goal, frozen, lvl[2], nlvl[2];
+    // load balancing
+    DBuf c_origin, load, samples, nsamp, pred;
+    MlpState mlp;
+    bool mlp_alloced = false;
 };
 
 static int fail(spz_ctx *c, int code, const char *fmt, ...) {
@@ -152,6 +157,7 @@ void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic) {
     cfg->lb_policy = SPZ_LB_UNIFORM;
     cfg->device = 0;
     cfg->time_kernels = 0;
+    cfg->log_samples = 0;
     cfg->queue_items = 0;
 }
 
@@ -220,6 +226,7 @@ int spz_create(spz_ctx **out, const spz_config *cfg) {
             (e = ctx->c_root.ensure(L * 4 * 4)) != hipSuccess || (e = ctx->c_meta.ensure(L * 4)) != hipSuccess ||
             (e = ctx->c_rmeta.ensure(L * 4)) != hipSuccess || (e = ctx->c_inst.ensure(L * 4)) != hipSuccess ||
             (e = ctx->c_cnt.ensure(L * 4)) != hipSuccess || (e = ctx->c_pdb.ensure(L * 4 * 10)) != hipSuccess ||
+            (e = ctx->c_origin.ensure(L * 4)) != hipSuccess ||
             (e = ctx->c_stack.ensure(L * (size_t)maxd)) != hipSuccess || (e = ctx->q_ctl.ensure(64)) != hipSuccess ||
             (e = ctx->ctl.ensure(64 * 4)) != hipSuccess) {
             rc = fail(ctx, SPZ_E_HIP, "context allocation: %s", hipGetErrorString(e));
@@ -247,8 +254,11 @@ void spz_destroy(spz_ctx *ctx) {
                    &ctx->cnt,         &ctx->pend,           &ctx->found,          &ctx->done,
                    &ctx->opt,         &ctx->iter_counts,    &ctx->restart_list,   &ctx->frontier,
                    &ctx->nfront,      &ctx->topgen,         &ctx->top_goal,       &ctx->frozen,
-                   &ctx->lvl[0],      &ctx->lvl[1],         &ctx->nlvl[0],        &ctx->nlvl[1]};
+                   &ctx->lvl[0],      &ctx->lvl[1],         &ctx->nlvl[0],        &ctx->nlvl[1],
+                   &ctx->c_origin,    &ctx->load,           &ctx->samples,        &ctx->nsamp,
+                   &ctx->pred};
     for (DBuf *b : all) b->release();
+    if (ctx->mlp_alloced) mlp_free(ctx->mlp);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -261,7 +271,7 @@ int spz_last_stats(spz_ctx *ctx, double *stats) {
 
 int spz_set_lb(spz_ctx *ctx, int32_t lb_policy) {
     if (!ctx) return SPZ_E_ARG;
-    if (lb_policy != SPZ_LB_UNIFORM && lb_policy != SPZ_LB_LEARNED)
+    if (lb_policy != SPZ_LB_UNIFORM && lb_policy != SPZ_LB_LEARNED && lb_policy != SPZ_LB_PREVLOAD)
         return fail(ctx, SPZ_E_ARG, "unknown lb policy %d", lb_policy);
     ctx->cfg.lb_policy = lb_policy;
     return SPZ_OK;
@@ -395,6 +405,13 @@ static KParams base_params(spz_ctx *ctx, int n, int max_iters, int capl) {
     p.c_rmeta = ctx->c_rmeta.as<uint32_t>();
     p.c_inst = ctx->c_inst.as<uint32_t>();
     p.c_cnt = ctx->c_cnt.as<uint32_t>();
+    p.c_origin = ctx->c_origin.as<uint32_t>();
+    p.load = ctx->load.as<unsigned long long>();
+    p.lb = ctx->cfg.lb_policy;
+    p.pred = ctx->pred.as<float>();
+    p.samples = ctx->samples.as<Sample>();
+    p.n_samples = ctx->nsamp.as<unsigned>();
+    p.max_samples = ctx->cfg.log_samples > 0 && ctx->samples.p ? (unsigned)ctx->cfg.log_samples : 0u;
     p.c_pdb = ctx->c_pdb.as<uint32_t>();
     p.c_stack = ctx->c_stack.as<uint8_t>();
     p.pdb0 = ctx->pdb_direct[0].as<uint8_t>();
@@ -474,6 +491,14 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     HIPCHK(ctx->topgen.ensure(N * kFBins * 4));
     HIPCHK(ctx->top_goal.ensure(N * 4));
     HIPCHK(ctx->frozen.ensure(N * 4));
+    HIPCHK(ctx->load.ensure(front_items * 8));
+    HIPCHK(ctx->nsamp.ensure(64));
+    if (ctx->cfg.log_samples > 0) HIPCHK(ctx->samples.ensure((size_t)ctx->cfg.log_samples * sizeof(Sample)));
+    if (ctx->cfg.lb_policy == SPZ_LB_LEARNED) {
+        if (!ctx->mlp_alloced || !ctx->mlp.ready)
+            return fail(ctx, SPZ_E_ARG, "learned load balancing needs MLP parameters (spz_mlp_set_params/train)");
+        HIPCHK(ctx->pred.ensure(front_items * kPredK * 4));
+    }
     for (int k = 0; k < 2; ++k) {
         HIPCHK(ctx->lvl[k].ensure(front_items * sizeof(Item)));
         HIPCHK(ctx->nlvl[k].ensure(N * 4));
@@ -514,6 +539,11 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         cur ^= 1;
     }
 
+    HIPCHK(hipMemsetAsync(ctx->load.p, 0, front_items * 8, s));
+    HIPCHK(hipMemsetAsync(ctx->nsamp.p, 0, 64, s));
+    if (ctx->cfg.lb_policy == SPZ_LB_LEARNED)
+        HIPCHK(mlp_predict_frontier(ctx->mlp, w, ctx->frontier.as<Item>(), ctx->nfront.as<unsigned>(),
+                                    ctx->roots.as<Item>(), n, capl, kPredK, ctx->pred.as<float>(), s));
     HIPCHK(hipStreamSynchronize(s));
     ctx->stats[10] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_split).count();
 
@@ -679,6 +709,192 @@ int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, double *ms, double *gather
     if (e != hipSuccess) return fail(ctx, SPZ_E_HIP, "gather probe: %s", hipGetErrorString(e));
     *ms = t;
     *gathers_per_s = (double)blocks * 256.0 * per_thread / (t * 1e-3);
+    return SPZ_OK;
+}
+
+// ------------------------------------------------------------ learned LB ---
+int32_t spz_mlp_param_count(void) { return (int32_t)kMlpParams; }
+
+static int mlp_ready_alloc(spz_ctx *ctx) {
+    if (!ctx->mlp_alloced) {
+        HIPCHK(mlp_alloc(ctx->mlp));
+        ctx->mlp_alloced = true;
+    }
+    return SPZ_OK;
+}
+
+int spz_mlp_init(spz_ctx *ctx, uint64_t seed) {
+    if (!ctx) return SPZ_E_ARG;
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    int rc = mlp_ready_alloc(ctx);
+    if (rc) return rc;
+    HIPCHK(mlp_init_random(ctx->mlp, seed, ctx->stream));
+    return SPZ_OK;
+}
+
+int spz_mlp_set_params(spz_ctx *ctx, const float *params) {
+    if (!ctx || !params) return SPZ_E_ARG;
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    int rc = mlp_ready_alloc(ctx);
+    if (rc) return rc;
+    HIPCHK(mlp_set_params(ctx->mlp, params, ctx->stream));
+    return SPZ_OK;
+}
+
+int spz_mlp_get_params(spz_ctx *ctx, float *params) {
+    if (!ctx || !params) return SPZ_E_ARG;
+    if (!ctx->mlp_alloced || !ctx->mlp.ready) return fail(ctx, SPZ_E_ARG, "no MLP parameters");
+    HIPCHK(mlp_get_params(ctx->mlp, params, ctx->stream));
+    return SPZ_OK;
+}
+
+// boards -> Items with the context's heuristic in meta -> feature rows
+static int mlp_load_rows(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, int32_t n, DBuf &items,
+                         DBuf &thr_d) {
+    const int w = ctx->cfg.width;
+    if (ctx->cfg.heuristic == SPZ_H_PDB666 && !ctx->have_pdb) return fail(ctx, SPZ_E_NOPDB, "call spz_build_pdb first");
+    for (int i = 0; i < n; ++i)
+        if (!is_solvable(w, boards + (size_t)i * w * w))
+            return fail(ctx, SPZ_E_ARG, "board %d is not a solvable permutation", i);
+    std::vector<Item> it((size_t)n);
+    for (int i = 0; i < n; ++i) pack_root(w, boards + (size_t)i * w * w, i, &it[i]);
+    HIPCHK(items.ensure((size_t)n * sizeof(Item)));
+    HIPCHK(thr_d.ensure((size_t)n * 4));
+    HIPCHK(hipMemcpyAsync(items.p, it.data(), (size_t)n * sizeof(Item), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(thr_d.p, thr, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    KParams p = base_params(ctx, 0, 1, 1);
+    HIPCHK(launch_eval_h(w, ctx->cfg.heuristic, p, ctx->geo_d.as<PdbGeo>(), items.as<Item>(), n, ctx->stream));
+    HIPCHK(mlp_reserve(ctx->mlp, (size_t)n));
+    HIPCHK(mlp_features_items(ctx->mlp, w, items.as<Item>(), thr_d.as<int>(), n, ctx->stream));
+    return SPZ_OK;
+}
+
+int spz_mlp_forward(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, int32_t n, float *out) {
+    if (!ctx || !boards || !thr || !out || n <= 0) return SPZ_E_ARG;
+    if (!ctx->mlp_alloced || !ctx->mlp.ready) return fail(ctx, SPZ_E_ARG, "no MLP parameters");
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    DBuf items, thr_d;
+    int rc = mlp_load_rows(ctx, boards, thr, n, items, thr_d);
+    if (rc) return rc;
+    HIPCHK(mlp_forward(ctx->mlp, n, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, ctx->mlp.y, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return SPZ_OK;
+}
+
+int spz_mlp_grad(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const float *target, int32_t n,
+                 float *loss, float *grad) {
+    if (!ctx || !boards || !thr || !target || !loss || !grad || n <= 0) return SPZ_E_ARG;
+    if (!ctx->mlp_alloced || !ctx->mlp.ready) return fail(ctx, SPZ_E_ARG, "no MLP parameters");
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    DBuf items, thr_d, tgt;
+    int rc = mlp_load_rows(ctx, boards, thr, n, items, thr_d);
+    if (rc) return rc;
+    HIPCHK(tgt.ensure((size_t)n * 4));
+    HIPCHK(hipMemcpyAsync(tgt.p, target, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(mlp_forward(ctx->mlp, n, ctx->stream));
+    HIPCHK(mlp_backward(ctx->mlp, tgt.as<float>(), n, ctx->stream));
+    HIPCHK(hipMemcpyAsync(loss, ctx->mlp.loss, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(grad, ctx->mlp.grad, kMlpParams * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return SPZ_OK;
+}
+
+int spz_mlp_train(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const float *target, int32_t n,
+                  int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss) {
+    if (!ctx || !boards || !thr || !target || n <= 0 || epochs <= 0 || batch <= 0) return SPZ_E_ARG;
+    if (!ctx->mlp_alloced || !ctx->mlp.ready) return fail(ctx, SPZ_E_ARG, "initialise the MLP first");
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    const int w = ctx->cfg.width;
+    hipStream_t s = ctx->stream;
+    // all rows to the device once; batches are gathered by a permutation
+    std::vector<Item> it((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        if (!is_solvable(w, boards + (size_t)i * w * w))
+            return fail(ctx, SPZ_E_ARG, "board %d is not a solvable permutation", i);
+        pack_root(w, boards + (size_t)i * w * w, i, &it[i]);
+    }
+    DBuf items, thr_all, tgt_all, bitems, bthr, btgt;
+    HIPCHK(items.ensure((size_t)n * sizeof(Item)));
+    HIPCHK(hipMemcpyAsync(items.p, it.data(), (size_t)n * sizeof(Item), hipMemcpyHostToDevice, s));
+    KParams p = base_params(ctx, 0, 1, 1);
+    HIPCHK(launch_eval_h(w, ctx->cfg.heuristic, p, ctx->geo_d.as<PdbGeo>(), items.as<Item>(), n, s));
+    HIPCHK(hipMemcpyAsync(it.data(), items.p, (size_t)n * sizeof(Item), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(bitems.ensure((size_t)batch * sizeof(Item)));
+    HIPCHK(bthr.ensure((size_t)batch * 4));
+    HIPCHK(btgt.ensure((size_t)batch * 4));
+    HIPCHK(mlp_reserve(ctx->mlp, (size_t)batch));
+    std::vector<int> perm((size_t)n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    uint64_t x = seed;
+    std::vector<Item> bi((size_t)batch);
+    std::vector<int> bt((size_t)batch);
+    std::vector<float> bg((size_t)batch);
+    for (int ep = 0; ep < epochs; ++ep) {
+        for (int i = n - 1; i > 0; --i) { // Fisher-Yates with splitmix64
+            uint64_t z = (x += 0x9e3779b97f4a7c15ULL);
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+            z ^= z >> 31;
+            std::swap(perm[i], perm[(int)(z % (uint64_t)(i + 1))]);
+        }
+        double sum = 0;
+        int nb = 0;
+        for (int b0 = 0; b0 + batch <= n || (b0 == 0 && n < batch); b0 += batch) {
+            int nbat = std::min(batch, n - b0);
+            for (int j = 0; j < nbat; ++j) {
+                bi[j] = it[perm[b0 + j]];
+                bt[j] = thr[perm[b0 + j]];
+                bg[j] = target[perm[b0 + j]];
+            }
+            HIPCHK(hipMemcpyAsync(bitems.p, bi.data(), (size_t)nbat * sizeof(Item), hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(bthr.p, bt.data(), (size_t)nbat * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(btgt.p, bg.data(), (size_t)nbat * 4, hipMemcpyHostToDevice, s));
+            HIPCHK(mlp_features_items(ctx->mlp, w, bitems.as<Item>(), bthr.as<int>(), nbat, s));
+            HIPCHK(mlp_forward(ctx->mlp, nbat, s));
+            HIPCHK(mlp_backward(ctx->mlp, btgt.as<float>(), nbat, s));
+            HIPCHK(mlp_adam_step(ctx->mlp, lr, s));
+            float l = 0;
+            HIPCHK(hipMemcpyAsync(&l, ctx->mlp.loss, 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            sum += l;
+            ++nb;
+        }
+        if (epoch_loss) epoch_loss[ep] = nb ? (float)(sum / nb) : 0.f;
+    }
+    return SPZ_OK;
+}
+
+int spz_get_samples(spz_ctx *ctx, uint8_t *boards, int32_t *thr, uint64_t *loads, int32_t max, int32_t *n_out) {
+    if (!ctx || !n_out) return SPZ_E_ARG;
+    *n_out = 0;
+    if (!ctx->samples.p || !ctx->nsamp.p) return SPZ_OK;
+    unsigned cnt = 0;
+    HIPCHK(hipMemcpy(&cnt, ctx->nsamp.p, 4, hipMemcpyDeviceToHost));
+    unsigned have = std::min<unsigned>(cnt, (unsigned)std::max(ctx->cfg.log_samples, 0));
+    unsigned n = std::min<unsigned>(have, (unsigned)std::max(max, 0));
+    if (n == 0 || !boards) {
+        *n_out = (int32_t)have;
+        return SPZ_OK;
+    }
+    std::vector<Sample> sm(n);
+    HIPCHK(hipMemcpy(sm.data(), ctx->samples.p, (size_t)n * sizeof(Sample), hipMemcpyDeviceToHost));
+    const int w = ctx->cfg.width, n2 = w * w;
+    for (unsigned i = 0; i < n; ++i) {
+        if (w == 4) {
+            Board<4> bd;
+            bd.unpack(sm[i].item.b);
+            for (int c = 0; c < n2; ++c) boards[(size_t)i * n2 + c] = (uint8_t)bd.tile(c);
+        } else {
+            Board<5> bd;
+            bd.unpack(sm[i].item.b);
+            for (int c = 0; c < n2; ++c) boards[(size_t)i * n2 + c] = (uint8_t)bd.tile(c);
+        }
+        if (thr) thr[i] = sm[i].thr;
+        if (loads) loads[i] = sm[i].load;
+    }
+    *n_out = (int32_t)n;
     return SPZ_OK;
 }
 

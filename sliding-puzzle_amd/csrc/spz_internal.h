@@ -21,14 +21,25 @@ constexpr int kNoGoal = 0x7fffffff; // found-depth sentinel
 // A search node in the work queue (32 bytes).  The board is packed:
 //  width 4: b[0] = row0 | row1<<16, b[1] = row2 | row3<<16 (4-bit cells)
 //  width 5: b[r] = row r (25 bits) | 7 bits of row 4 at bit 25 (r = 0..3)
-// meta = blank | g<<8 | h<<16 | prev<<24; x0/x1 carry the eight PDB lookup
-// values (6 bits each) so that a lane starting the item needs no gathers.
+// meta = blank | g<<8 | h<<16 | prev<<24; x0 = largest f on the root path
+// (root split), x1 = index of the frontier node the item descends from.
 struct alignas(16) Item {
     uint32_t b[4];
     uint32_t meta;
     uint32_t inst;
     uint32_t x0, x1;
 };
+
+// A logged training sample of the learned load balancer: a frontier node,
+// the threshold of an iteration, and the nodes generated below it then.
+struct Sample {
+    Item item;
+    int32_t thr;
+    uint32_t pad;
+    unsigned long long load;
+};
+
+constexpr int kPredK = 16; // thresholds predicted per frontier node
 
 __host__ __device__ inline uint32_t meta_pack(int blank, int g, int h, int prev) {
     return (uint32_t)blank | ((uint32_t)g << 8) | ((uint32_t)h << 16) | ((uint32_t)prev << 24);
@@ -70,6 +81,15 @@ struct KParams {
     const int *top_goal;         // [n_inst] min depth of a goal in the top tree
     int capl;
 
+    // load balancing: nodes generated below each frontier node in the
+    // current iteration, dispatch policy, learned predictions, sample log
+    unsigned long long *load;    // [n_inst][capl]
+    int lb;                      // 0 frontier order, 1 learned, 2 previous load
+    const float *pred;           // [n_inst][capl][kPredK] predicted log2(1+load)
+    Sample *samples;
+    unsigned *n_samples;
+    unsigned max_samples;
+
     // lane contexts, structure of arrays over n_lanes lanes
     int n_lanes;
     int maxd;          // LDS stack levels per lane in use
@@ -80,6 +100,7 @@ struct KParams {
     uint32_t *c_rmeta; // meta of the item root
     uint32_t *c_inst;
     uint32_t *c_cnt;
+    uint32_t *c_origin; // frontier node the lane's item descends from
     uint32_t *c_pdb;   // [10][n_lanes]: 8 lookup indices + 2 packed value words
     uint8_t *c_stack;  // [maxd][n_lanes]
 

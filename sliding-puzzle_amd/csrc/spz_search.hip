@@ -61,13 +61,13 @@ __device__ void finish_iteration(const KParams &p, int inst) {
 // Append one item.  pos is a reserved queue position.
 template <int W, int HEUR>
 __device__ __forceinline__ void put_item(const KParams &p, unsigned long long pos, const Pos<W, HEUR> &s, int g,
-                                         int prev, int inst) {
+                                         int prev, int inst, uint32_t origin) {
     Item it;
     s.bd.pack(it.b);
     it.meta = meta_pack(s.br * W + s.bc, g, s.hs.h, prev);
     it.inst = (uint32_t)inst;
     it.x0 = 0;
-    it.x1 = 0;
+    it.x1 = origin;
     p.q[pos & p.qmask] = it;
 }
 
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
     uint32_t root[4] = {0, 0, 0, 0};
     uint32_t rmeta = 0;
     int g = 0, prev = DIR_NONE, inst = 0, L = 0;
-    uint32_t cnt = 0;
+    uint32_t cnt = 0, origin = 0;
 
     if (st == ST_FRESH || st == ST_POP) {
         uint32_t b[4];
@@ -136,6 +136,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
         rmeta = p.c_rmeta[gl];
         inst = (int)p.c_inst[gl];
         cnt = p.c_cnt[gl];
+        origin = p.c_origin[gl];
         int blank = meta_blank(m);
         s.br = blank / W;
         s.bc = blank - s.br * W;
@@ -181,6 +182,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                     for (int k = 0; k < 4; ++k) root[k] = it.b[k];
                     rmeta = it.meta;
                     inst = (int)it.inst;
+                    origin = it.x1;
                     g = meta_g(it.meta);
                     prev = meta_prev(it.meta);
                     L = p.L[inst];
@@ -220,7 +222,10 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
             }
         }
         if (finish) {
-            if (cnt) atomicAdd(&p.cnt[inst], (unsigned long long)cnt);
+            if (cnt) {
+                atomicAdd(&p.cnt[inst], (unsigned long long)cnt);
+                atomicAdd(&p.load[(size_t)inst * p.capl + origin], (unsigned long long)cnt);
+            }
             __threadfence();
             unsigned old = atomicSub(&p.pend[inst], 1u);
             if (old == 1u) finish_iteration(p, inst);
@@ -311,7 +316,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                             pass &= pass - 1;
                             Pos<W, HEUR> c = s;
                             do_move<W, HEUR>(c, d, P);
-                            put_item<W, HEUR>(p, pos++, c, g + 1, d, inst);
+                            put_item<W, HEUR>(p, pos++, c, g + 1, d, inst, origin);
                         }
                         st = ST_POP;
                     }
@@ -363,7 +368,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                     R &= R - 1;
                     Pos<W, HEUR> x = c;
                     do_move<W, HEUR>(x, d2, P);
-                    put_item<W, HEUR>(p, pos++, x, cg + 1, d2, inst);
+                    put_item<W, HEUR>(p, pos++, x, cg + 1, d2, inst, origin);
                     ++emitted;
                 }
                 stk[k * sstride] = (uint8_t)((R << 2) | (uint32_t)d);
@@ -378,6 +383,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
     // keep the per-lane u32 count far from wrapping on long-lived items
     if (busy && cnt >= (1u << 30)) {
         atomicAdd(&p.cnt[inst], (unsigned long long)cnt);
+        atomicAdd(&p.load[(size_t)inst * p.capl + origin], (unsigned long long)cnt);
         cnt = 0;
     }
     uint64_t busy_mask = __ballot(busy);
@@ -394,6 +400,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
         p.c_rmeta[gl] = rmeta;
         p.c_inst[gl] = (uint32_t)inst;
         p.c_cnt[gl] = cnt;
+        p.c_origin[gl] = origin;
         if constexpr (HEUR == 1) {
             p.c_pdb[0 * NL + gl] = s.hs.i0;
             p.c_pdb[1 * NL + gl] = s.hs.i1;
@@ -420,12 +427,33 @@ __device__ __forceinline__ int path_f(const Item &it) {
 
 // ---------------------------------------------------------------------------
 // k_restart: start the current iteration of every instance on the restart
-// list: add the top-tree count from the f-histogram, note goals in the top
-// tree, and queue the frontier nodes whose f fits.  An iteration with no
-// frontier work completes here.  One block per listed instance.
+// list (one block each).  It
+//   1. logs (frontier node, previous threshold, nodes generated below it)
+//      samples of the iteration that just ended (learned-LB training data),
+//   2. adds the top-tree count of the new threshold from the f-histogram and
+//      notes goals in the top tree,
+//   3. queues the frontier nodes whose path-f fits, ordered by a key in
+//      decreasing order (counting sort over 64 half-log2 buckets): frontier
+//      order (uniform), last iteration's load (the reference's load-based
+//      static LB, src/15puzzle_block_parallel.cu:490-563), or the learned
+//      prediction -- longest expected subtree first,
+//   4. clears the per-frontier-node loads.
+// An iteration with no frontier work completes here.
+__device__ __forceinline__ int lb_bucket(const KParams &p, int inst, unsigned i, int it) {
+    float key = 0.f;
+    if (p.lb == 2) {
+        key = it > 0 ? log2f((float)p.load[(size_t)inst * p.capl + i] + 1.f) : 0.f;
+    } else if (p.lb == 1) {
+        key = p.pred[((size_t)inst * p.capl + i) * kPredK + (it < kPredK ? it : kPredK - 1)];
+    }
+    int b = (int)(key * 2.f);
+    return b < 0 ? 0 : (b > 63 ? 63 : b);
+}
+
 __global__ void __launch_bounds__(256) k_restart(KParams p) {
     __shared__ unsigned long long s_cnt;
     __shared__ unsigned s_n;
+    __shared__ unsigned s_bucket[64];
     __shared__ unsigned long long s_base;
     __shared__ int s_more;
     const unsigned n_list = p.ctl[0];
@@ -433,46 +461,71 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
     const int inst = p.restart_list[blockIdx.x];
     const unsigned nf = min(p.nfront[inst], (unsigned)p.capl);
     const Item *F = p.frontier + (size_t)inst * p.capl;
+    unsigned long long *load = p.load + (size_t)inst * p.capl;
 
     for (;;) {
         const int L = p.L[inst];
+        const int it = p.it[inst];
+        // 1. samples of the iteration that ended
+        if (it > 0 && p.max_samples > 0) {
+            for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) {
+                if (path_f(F[i]) <= L - 2) {
+                    unsigned k = atomicAdd(p.n_samples, 1u);
+                    if (k < p.max_samples) {
+                        Sample sm;
+                        sm.item = F[i];
+                        sm.thr = L - 2;
+                        sm.pad = 0;
+                        sm.load = load[i];
+                        p.samples[k] = sm;
+                    }
+                }
+            }
+        }
         if (threadIdx.x == 0) {
             s_cnt = 0;
             s_n = 0;
         }
+        for (int b = threadIdx.x; b < 64; b += blockDim.x) s_bucket[b] = 0;
         __syncthreads();
-        // top tree: sum of gen over f <= L
+        // 2. top tree: sum of gen over path-f <= L
         unsigned long long c = 0;
         for (int f = threadIdx.x; f < kFBins; f += blockDim.x)
             if (f <= L) c += p.topgen[(size_t)inst * kFBins + f];
         atomicAdd(&s_cnt, c);
-        // frontier nodes that fit
-        unsigned mine = 0;
+        // 3. bucket sizes of the frontier nodes that fit
         for (unsigned i = threadIdx.x; i < nf; i += blockDim.x)
-            if (path_f(F[i]) <= L) ++mine;
-        atomicAdd(&s_n, mine);
+            if (path_f(F[i]) <= L) atomicAdd(&s_bucket[lb_bucket(p, inst, i, it)], 1u);
         __syncthreads();
         if (threadIdx.x == 0) {
-            s_base = s_n ? atomicAdd(&p.q_ctl[2], (unsigned long long)s_n) : 0ull;
+            unsigned off = 0;
+            for (int b = 63; b >= 0; --b) { // largest keys first
+                unsigned n = s_bucket[b];
+                s_bucket[b] = off;
+                off += n;
+            }
+            s_n = off;
+            s_base = off ? atomicAdd(&p.q_ctl[2], (unsigned long long)off) : 0ull;
             if (p.top_goal[inst] <= L) atomicMin(&p.found[inst], p.top_goal[inst]);
             p.cnt[inst] = s_cnt;
-            p.pend[inst] = s_n;
-            s_n = 0; // reused as the write cursor
+            p.pend[inst] = off;
         }
         __syncthreads();
         for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) {
-            Item it = F[i];
-            if (path_f(it) <= L) {
-                unsigned k = atomicAdd(&s_n, 1u);
-                p.q[(s_base + k) & p.qmask] = it;
+            Item x = F[i];
+            if (path_f(x) <= L) {
+                unsigned k = atomicAdd(&s_bucket[lb_bucket(p, inst, i, it)], 1u);
+                x.x1 = i;
+                p.q[(s_base + k) & p.qmask] = x;
             }
         }
         __syncthreads();
+        // 4. loads of the new iteration start from zero
+        for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) load[i] = 0;
         if (threadIdx.x == 0) {
             s_more = 0;
-            if (p.pend[inst] == 0) {
+            if (s_n == 0) {
                 // nothing to search below the split: the iteration is complete
-                int it = p.it[inst];
                 p.iter_counts[(size_t)inst * p.max_iters + it] = s_cnt;
                 if (p.found[inst] != kNoGoal) {
                     p.opt[inst] = p.found[inst];

@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libspz.so")
 
 H_MANHATTAN, H_PDB666 = 0, 1
 MODE_ALL, MODE_FIRST = 0, 1
-LB_UNIFORM, LB_LEARNED = 0, 1
+LB_UNIFORM, LB_LEARNED, LB_PREVLOAD = 0, 1, 2
 MAX_ITERS = 128
 
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "no PDB", -4: "queue capacity",
@@ -45,7 +45,8 @@ class Config(ctypes.Structure):
                 ("lanes", ctypes.c_int32), ("round_budget", ctypes.c_int32),
                 ("lds_stack_depth", ctypes.c_int32),
                 ("lb_policy", ctypes.c_int32), ("device", ctypes.c_int32),
-                ("time_kernels", ctypes.c_int32), ("queue_items", ctypes.c_int64)]
+                ("time_kernels", ctypes.c_int32), ("log_samples", ctypes.c_int32),
+                ("queue_items", ctypes.c_int64)]
 
 
 class _Result(ctypes.Structure):
@@ -71,7 +72,9 @@ _lib = None
 EXPORTS = ["spz_default_config", "spz_create", "spz_build_pdb", "spz_export_pdb_tab",
            "spz_get_pdb", "spz_solve_batch", "spz_eval_h", "spz_set_lb", "spz_last_stats",
            "spz_last_error", "spz_destroy", "spz_pdb_geometry", "spz_gen_walk",
-           "spz_gen_random", "spz_gather_probe"]
+           "spz_gen_random", "spz_gather_probe", "spz_mlp_param_count", "spz_mlp_init",
+           "spz_mlp_set_params", "spz_mlp_get_params", "spz_mlp_forward", "spz_mlp_grad",
+           "spz_mlp_train", "spz_get_samples"]
 
 
 def lib() -> ctypes.CDLL:
@@ -100,6 +103,17 @@ def lib() -> ctypes.CDLL:
     L.spz_destroy.restype = None
     L.spz_gather_probe.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_double)]
+    f32p, i32p = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)
+    L.spz_mlp_param_count.argtypes = []
+    L.spz_mlp_param_count.restype = ctypes.c_int32
+    L.spz_mlp_init.argtypes = [vp, ctypes.c_uint64]
+    L.spz_mlp_set_params.argtypes = [vp, f32p]
+    L.spz_mlp_get_params.argtypes = [vp, f32p]
+    L.spz_mlp_forward.argtypes = [vp, u8p, i32p, ctypes.c_int32, f32p]
+    L.spz_mlp_grad.argtypes = [vp, u8p, i32p, f32p, ctypes.c_int32, f32p, f32p]
+    L.spz_mlp_train.argtypes = [vp, u8p, i32p, f32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                ctypes.c_float, ctypes.c_uint64, f32p]
+    L.spz_get_samples.argtypes = [vp, u8p, i32p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int32, i32p]
     L.spz_pdb_geometry.argtypes = [u8p, u8p]
     L.spz_pdb_geometry.restype = None
     L.spz_gen_walk.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, u8p]
@@ -159,7 +173,7 @@ class Solver:
                  frontier_target: int = 2048, max_split_depth: int = 24,
                  workgroup_size: int = 256, lanes: int = 0, round_budget: int = 0,
                  lb_policy: int = LB_UNIFORM, device: int = 0, queue_items: int = 0,
-                 time_kernels: bool = False, lds_stack_depth: int = 0):
+                 time_kernels: bool = False, lds_stack_depth: int = 0, log_samples: int = 0):
         L = lib()
         cfg = Config()
         L.spz_default_config(ctypes.byref(cfg), width, heuristic)
@@ -174,6 +188,7 @@ class Solver:
         cfg.queue_items = queue_items
         cfg.time_kernels = 1 if time_kernels else 0
         cfg.lds_stack_depth = lds_stack_depth
+        cfg.log_samples = log_samples
         self.cfg = cfg
         self.width = width
         self._ctx = ctypes.c_void_p()
@@ -225,6 +240,77 @@ class Solver:
         self._check(lib().spz_gather_probe(self._ctx, n_gathers, ctypes.byref(ms), ctypes.byref(gps)),
                     "spz_gather_probe")
         return ms.value, gps.value
+
+    # ---- learned load balancer -------------------------------------------
+    @staticmethod
+    def mlp_param_count() -> int:
+        return int(lib().spz_mlp_param_count())
+
+    def mlp_init(self, seed: int = 1):
+        self._check(lib().spz_mlp_init(self._ctx, seed), "spz_mlp_init")
+
+    def mlp_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == self.mlp_param_count()
+        self._check(lib().spz_mlp_set_params(self._ctx, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "spz_mlp_set_params")
+
+    def mlp_get_params(self) -> np.ndarray:
+        p = np.zeros(self.mlp_param_count(), dtype=np.float32)
+        self._check(lib().spz_mlp_get_params(self._ctx, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "spz_mlp_get_params")
+        return p
+
+    def _rows(self, boards, thr):
+        b = np.ascontiguousarray(np.asarray(boards, dtype=np.uint8).reshape(-1, self.width ** 2))
+        t = np.ascontiguousarray(np.asarray(thr, dtype=np.int32).reshape(-1))
+        assert len(b) == len(t)
+        return b, t, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+    def mlp_forward(self, boards, thr) -> np.ndarray:
+        b, t, tp = self._rows(boards, thr)
+        out = np.zeros(len(b), dtype=np.float32)
+        self._check(lib().spz_mlp_forward(self._ctx, _u8p(b), tp, len(b),
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "spz_mlp_forward")
+        return out
+
+    def mlp_grad(self, boards, thr, target):
+        b, t, tp = self._rows(boards, thr)
+        tg = np.ascontiguousarray(target, dtype=np.float32)
+        loss = ctypes.c_float()
+        g = np.zeros(self.mlp_param_count(), dtype=np.float32)
+        self._check(lib().spz_mlp_grad(self._ctx, _u8p(b), tp,
+                                       tg.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(b),
+                                       ctypes.byref(loss),
+                                       g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "spz_mlp_grad")
+        return loss.value, g
+
+    def mlp_train(self, boards, thr, target, epochs=4, batch=1024, lr=1e-3, seed=1) -> np.ndarray:
+        b, t, tp = self._rows(boards, thr)
+        tg = np.ascontiguousarray(target, dtype=np.float32)
+        el = np.zeros(epochs, dtype=np.float32)
+        self._check(lib().spz_mlp_train(self._ctx, _u8p(b), tp,
+                                        tg.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(b),
+                                        epochs, batch, lr, seed,
+                                        el.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "spz_mlp_train")
+        return el
+
+    def get_samples(self):
+        n = ctypes.c_int32()
+        self._check(lib().spz_get_samples(self._ctx, None, None, None, 0, ctypes.byref(n)),
+                    "spz_get_samples")
+        k = n.value
+        b = np.zeros((max(k, 1), self.width ** 2), dtype=np.uint8)
+        t = np.zeros(max(k, 1), dtype=np.int32)
+        ld = np.zeros(max(k, 1), dtype=np.uint64)
+        self._check(lib().spz_get_samples(self._ctx, _u8p(b), t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                          ld.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), k,
+                                          ctypes.byref(n)), "spz_get_samples")
+        k = n.value
+        return b[:k], t[:k], ld[:k]
 
     def set_lb(self, policy: int):
         self._check(lib().spz_set_lb(self._ctx, policy), "spz_set_lb")
