@@ -92,6 +92,42 @@ def traffic_per_node(config: str):
     return None
 
 
+def train_predictor(solver, config, width, n_train=None):
+    """The learned load balancer's setup (not timed): solve a separate set of
+    training instances of the same configuration in frontier order with the
+    sample log on, train the MLP on the GPU on (frontier node, threshold,
+    log2(1 + subtree size)), and measure its held-out error against the
+    uniform predictor (one constant, the training mean)."""
+    t0 = time.perf_counter()
+    _, heur, tb = instances(config, n_train)
+    # training boards: seeds disjoint from the benchmark's (offset 10^7)
+    if config in ("c4", "c5"):
+        L = 160 if config == "c4" else 240
+        base = (4 if config == "c4" else 5) * 1_000_000 + 10_000_000
+        tb = [spz.gen_walk(5, L, base + i) for i in range(n_train or 64)]
+    logger = spz.Solver(width=width, heuristic=heur, frontier_target=FRONTIER[config],
+                        log_samples=2_000_000)
+    if heur == spz.H_PDB666:
+        logger.build_pdb()
+    logger.solve(np.stack(tb))
+    sb, st, sl = logger.get_samples()
+    logger.close()
+    y = np.log2(sl.astype(np.float64) + 1).astype(np.float32)
+    rng = np.random.default_rng(1)
+    idx = rng.permutation(len(sb))
+    cut = int(0.9 * len(idx))
+    tr, te = idx[:cut], idx[cut:]
+    solver.mlp_init(1)
+    losses = solver.mlp_train(sb[tr], st[tr], y[tr], epochs=6, batch=1024, lr=1e-3, seed=3)
+    pred = solver.mlp_forward(sb[te], st[te])
+    mse = float(np.mean((pred - y[te]) ** 2))
+    mse_u = float(np.mean((y[tr].mean() - y[te]) ** 2))
+    return {"samples": int(len(sb)), "train_instances": len(tb), "epochs": 6,
+            "epoch_loss": [round(float(v), 4) for v in losses],
+            "heldout_mse_log2": mse, "uniform_predictor_mse_log2": mse_u,
+            "setup_s": time.perf_counter() - t0}
+
+
 def solve_step(solver, boards, per_instance: bool, max_iters: int):
     if per_instance:
         out = []
@@ -180,6 +216,10 @@ def main():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--lds-depth", type=int, default=0)
     ap.add_argument("--wg", type=int, default=256)
+    ap.add_argument("--lb", default="learned", choices=["learned", "uniform", "prevload"],
+                    help="dispatch order of the timed solver (default: the learned predictor)")
+    ap.add_argument("--no-ablation", action="store_true",
+                    help="skip the one-step uniform/previous-load comparison solves")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one plain device-to-device copy of MIB MiB (a kernel of "
                          "known traffic that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -209,6 +249,13 @@ def main():
         t = time.time()
         solver.build_pdb()
         pdb_ms = (time.time() - t) * 1e3
+
+    learned = None
+    if args.lb == "learned":
+        learned = train_predictor(solver, args.config, width)
+        solver.set_lb(spz.LB_LEARNED)
+    elif args.lb == "prevload":
+        solver.set_lb(spz.LB_PREVLOAD)
 
     ref = None
     for _ in range(args.warmup):
@@ -258,7 +305,7 @@ def main():
     roof["launches"] = launches
 
     out = {
-        "metric": "instances/sec (+ nodes/sec), exact optimal length and per-iteration counts",
+        "metric": "instances/sec + nodes/sec, batched+learned-LB vs plain GPU IDA*, exact optimal length",
         "value": inst_per_s, "unit": "instances/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic (seeded)",
@@ -269,6 +316,20 @@ def main():
         "breakdown_per_step": dict(brk, round_kernel_ms=round_ms / args.steps,
                                    lanes=solver.stats()["lanes"]),
     }
+    out["lb"] = args.lb
+    if learned is not None:
+        out["learned_lb"] = learned
+    if not args.no_ablation and not per_instance:
+        # the same batch under the other dispatch orders (one step each)
+        for name, pol in (("uniform", spz.LB_UNIFORM), ("prevload", spz.LB_PREVLOAD)):
+            if name == args.lb:
+                continue
+            solver.set_lb(pol)
+            t = time.perf_counter()
+            r2 = solver.solve(boards, max_iters=args.max_iters)
+            dt = time.perf_counter() - t
+            assert [(r.opt_len, r.counts) for r in r2] == [(r.opt_len, r.counts) for r in ref]
+            out[f"{name}_lb_instances_per_s"] = len(boards) / dt
     if pdb_ms is not None:
         out["pdb_build_ms"] = pdb_ms
         _, gps = solver.gather_probe(1 << 30)

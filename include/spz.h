@@ -78,6 +78,8 @@ typedef struct spz_config {
     int32_t device;          /* HIP device ordinal */
     int32_t time_kernels;    /* 1: time every search-round launch with HIP
                                 events on the solver's stream (stats 8-9) */
+    int32_t use_graph;       /* 1: replay the search rounds from a captured
+                                hipGraph (default); 0: launch them one by one */
     int32_t log_samples;     /* >0: keep up to this many (frontier node,
                                 threshold, subtree count) samples per solve
                                 (spz_get_samples); training data of the MLP */
@@ -98,6 +100,23 @@ void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic);
 /* Create a context on cfg->device.  Replaces the per-process globals and
  * cudaMalloc of src/15puzzle_block_parallel.cu:424-452. */
 int spz_create(spz_ctx **ctx, const spz_config *cfg);
+
+/* Change the launch configuration of a context (workgroup size, lanes,
+ * LDS stack depth, root split, round budget, policy ...); width, heuristic
+ * and device stay fixed and built PDBs are kept.  No reference counterpart
+ * (the reference fixes these at compile time, src/15puzzle_block_parallel.cu:18-38). */
+int spz_reconfigure(spz_ctx *ctx, const spz_config *cfg);
+void spz_get_config(spz_ctx *ctx, spz_config *cfg);
+
+/* Autotuner (no reference counterpart; the reference's shared-memory sweep
+ * was done by hand, main.tex:742): coordinate descent over workgroup size
+ * 64/128/256, root-split frontier target (= split depth), LDS stack depth
+ * and round budget, timing spz_solve_batch on the sample boards.  Every
+ * candidate must reproduce the first one's lengths and per-iteration
+ * counts (SPZ_E_STALL otherwise).  The context keeps the fastest
+ * configuration, also returned in *best. */
+int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_iters, int32_t passes,
+                 spz_config *best, double *best_ms, int32_t *tried);
 
 /* Build the 6-6-6-6 pattern databases on the GPU by backward 0-1 BFS from
  * the goal (blank at cell 0).  Replaces loading Korf's pat24.*.tab files
@@ -165,7 +184,8 @@ int spz_get_samples(spz_ctx *ctx, uint8_t *boards, int32_t *thr, uint64_t *loads
  *  7 stack-overflow items  8 round-kernel ms (sum, when time_kernels)
  *  9 round launches timed  10 split ms            11 instances
  *  12 nodes (all instances, all iterations)
- *  13 setup ms (upload, root split, first restart)  14 rounds wall ms */
+ *  13 setup ms (upload, root split, first restart)  14 rounds wall ms
+ *  15 1 if the rounds ran from a hipGraph */
 int spz_last_stats(spz_ctx *ctx, double *stats);
 
 const char *spz_last_error(spz_ctx *ctx);

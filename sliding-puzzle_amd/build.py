@@ -12,6 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib", "libspz.so")
+CLI = os.path.join(HERE, "bin", "spz_solve")
 SOURCES = ["spz_api.hip", "spz_search.hip", "spz_pdb.hip", "spz_mlp.hip"]
 
 HIPCC_FLAGS = [
@@ -42,6 +43,17 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed")
     os.replace(tmp, LIB)
+    # the command-line front end: host C++ linked against libspz
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(os.path.dirname(HERE), "include"),
+           "-o", CLI, os.path.join(HERE, "host", "spz_solve.cpp"),
+           "-L", os.path.dirname(LIB), "-lspz", "-Wl,-rpath,$ORIGIN/../lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ (spz_solve) failed")
     return LIB
 
 

@@ -158,7 +158,48 @@ void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic) {
     cfg->device = 0;
     cfg->time_kernels = 0;
     cfg->log_samples = 0;
+    cfg->use_graph = 1;
     cfg->queue_items = 0;
+}
+
+// Lane count (from occupancy unless given) and lane-context arrays for the
+// context's workgroup size and LDS stack depth.
+static int configure_lanes(spz_ctx *ctx) {
+    const spz_config *cfg = &ctx->cfg;
+    const int wg = ctx->wg;
+    hipError_t e;
+    int lanes = cfg->lanes;
+    if (lanes <= 0) {
+        int blocks = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, round_kernel_ptr(cfg->width, cfg->heuristic), wg,
+                                                         round_lds_bytes(cfg->lds_stack_depth, wg));
+        if (e != hipSuccess || blocks <= 0) blocks = 2;
+        lanes = ctx->n_cu * blocks * wg;
+    }
+    lanes = (lanes + wg - 1) / wg * wg;
+    ctx->n_lanes = lanes;
+    size_t L = (size_t)lanes;
+    int maxd = cfg->lds_stack_depth;
+    if ((e = ctx->c_st.ensure(L * 4)) != hipSuccess || (e = ctx->c_rows.ensure(L * 4 * 5)) != hipSuccess ||
+        (e = ctx->c_root.ensure(L * 4 * 4)) != hipSuccess || (e = ctx->c_meta.ensure(L * 4)) != hipSuccess ||
+        (e = ctx->c_rmeta.ensure(L * 4)) != hipSuccess || (e = ctx->c_inst.ensure(L * 4)) != hipSuccess ||
+        (e = ctx->c_cnt.ensure(L * 4)) != hipSuccess || (e = ctx->c_pdb.ensure(L * 4 * 10)) != hipSuccess ||
+        (e = ctx->c_origin.ensure(L * 4)) != hipSuccess || (e = ctx->c_stack.ensure(L * (size_t)maxd)) != hipSuccess)
+        return fail(ctx, SPZ_E_HIP, "lane context allocation: %s", hipGetErrorString(e));
+    return SPZ_OK;
+}
+
+static int normalise_config(const spz_config *in, spz_config *c, int *wg) {
+    *c = *in;
+    *wg = c->workgroup_size ? c->workgroup_size : 256;
+    if (*wg != 64 && *wg != 128 && *wg != 256) return SPZ_E_ARG;
+    c->workgroup_size = *wg;
+    if (c->frontier_target < 1) c->frontier_target = 1;
+    if (c->max_split_depth <= 0) c->max_split_depth = 24;
+    if (c->lds_stack_depth <= 0) c->lds_stack_depth = c->width == 4 ? Geo<4>::DEFD : Geo<5>::DEFD;
+    if (c->lds_stack_depth < 8 || c->lds_stack_depth > Geo<5>::MAXD) return SPZ_E_ARG;
+    if (c->lb_policy < SPZ_LB_UNIFORM || c->lb_policy > SPZ_LB_PREVLOAD) return SPZ_E_ARG;
+    return SPZ_OK;
 }
 
 int spz_create(spz_ctx **out, const spz_config *cfg) {
@@ -168,21 +209,15 @@ int spz_create(spz_ctx **out, const spz_config *cfg) {
     if (cfg->width != 4 && cfg->width != 5) return SPZ_E_ARG;
     if (cfg->heuristic == SPZ_H_PDB666 && cfg->width != 5) return SPZ_E_ARG;
     if (cfg->heuristic != SPZ_H_MANHATTAN && cfg->heuristic != SPZ_H_PDB666) return SPZ_E_ARG;
-    int wg = cfg->workgroup_size ? cfg->workgroup_size : 256;
-    if (wg != 64 && wg != 128 && wg != 256) return SPZ_E_ARG;
+    spz_config ncfg;
+    int wg = 256;
+    if (normalise_config(cfg, &ncfg, &wg) != SPZ_OK) return SPZ_E_ARG;
     ctx = new spz_ctx();
-    ctx->cfg = *cfg;
+    ctx->cfg = ncfg;
     ctx->wg = wg;
     {
         const char *t = getenv("SPZ_TIMEOUT_S");
         ctx->timeout_s = t ? atof(t) : 900.0;
-    }
-    if (ctx->cfg.frontier_target < 1) ctx->cfg.frontier_target = 1;
-    if (ctx->cfg.max_split_depth <= 0) ctx->cfg.max_split_depth = 24;
-    if (ctx->cfg.lds_stack_depth <= 0) ctx->cfg.lds_stack_depth = cfg->width == 4 ? Geo<4>::DEFD : Geo<5>::DEFD;
-    if (ctx->cfg.lds_stack_depth < 8 || ctx->cfg.lds_stack_depth > Geo<5>::MAXD) {
-        delete ctx;
-        return SPZ_E_ARG;
     }
     int rc = SPZ_OK;
     do {
@@ -203,35 +238,14 @@ int spz_create(spz_ctx **out, const spz_config *cfg) {
             rc = fail(ctx, SPZ_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
             break;
         }
-        int lanes = cfg->lanes;
-        if (lanes <= 0) {
-            int blocks = 0;
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &blocks, round_kernel_ptr(cfg->width, cfg->heuristic), wg,
-                round_lds_bytes(ctx->cfg.lds_stack_depth, wg));
-            if (e != hipSuccess || blocks <= 0) blocks = 2;
-            lanes = ctx->n_cu * blocks * wg;
-        }
-        lanes = (lanes + wg - 1) / wg * wg;
-        ctx->n_lanes = lanes;
         pdb_geo_build(&ctx->geo_h);
         if ((e = ctx->geo_d.ensure(sizeof(PdbGeo))) != hipSuccess ||
-            (e = hipMemcpy(ctx->geo_d.p, &ctx->geo_h, sizeof(PdbGeo), hipMemcpyHostToDevice)) != hipSuccess) {
-            rc = fail(ctx, SPZ_E_HIP, "geometry upload: %s", hipGetErrorString(e));
-            break;
-        }
-        size_t L = (size_t)lanes;
-        int maxd = ctx->cfg.lds_stack_depth;
-        if ((e = ctx->c_st.ensure(L * 4)) != hipSuccess || (e = ctx->c_rows.ensure(L * 4 * 5)) != hipSuccess ||
-            (e = ctx->c_root.ensure(L * 4 * 4)) != hipSuccess || (e = ctx->c_meta.ensure(L * 4)) != hipSuccess ||
-            (e = ctx->c_rmeta.ensure(L * 4)) != hipSuccess || (e = ctx->c_inst.ensure(L * 4)) != hipSuccess ||
-            (e = ctx->c_cnt.ensure(L * 4)) != hipSuccess || (e = ctx->c_pdb.ensure(L * 4 * 10)) != hipSuccess ||
-            (e = ctx->c_origin.ensure(L * 4)) != hipSuccess ||
-            (e = ctx->c_stack.ensure(L * (size_t)maxd)) != hipSuccess || (e = ctx->q_ctl.ensure(64)) != hipSuccess ||
-            (e = ctx->ctl.ensure(64 * 4)) != hipSuccess) {
+            (e = hipMemcpy(ctx->geo_d.p, &ctx->geo_h, sizeof(PdbGeo), hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = ctx->q_ctl.ensure(64)) != hipSuccess || (e = ctx->ctl.ensure(64 * 4)) != hipSuccess) {
             rc = fail(ctx, SPZ_E_HIP, "context allocation: %s", hipGetErrorString(e));
             break;
         }
+        rc = configure_lanes(ctx);
     } while (0);
     if (rc != SPZ_OK) {
         // keep the context so the caller can read the error, but report failure
@@ -239,6 +253,97 @@ int spz_create(spz_ctx **out, const spz_config *cfg) {
         return rc;
     }
     *out = ctx;
+    return SPZ_OK;
+}
+
+int spz_reconfigure(spz_ctx *ctx, const spz_config *cfg) {
+    if (!ctx || !cfg) return SPZ_E_ARG;
+    if (cfg->width != ctx->cfg.width || cfg->heuristic != ctx->cfg.heuristic || cfg->device != ctx->cfg.device)
+        return fail(ctx, SPZ_E_ARG, "width, heuristic and device are fixed for a context");
+    spz_config ncfg;
+    int wg = 256;
+    if (normalise_config(cfg, &ncfg, &wg) != SPZ_OK) return fail(ctx, SPZ_E_ARG, "invalid configuration");
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    ctx->cfg = ncfg;
+    ctx->wg = wg;
+    return configure_lanes(ctx);
+}
+
+void spz_get_config(spz_ctx *ctx, spz_config *cfg) {
+    if (ctx && cfg) *cfg = ctx->cfg;
+}
+
+// Coordinate descent over workgroup size, root-split target, LDS stack depth
+// and round budget on a sample batch; every candidate must give the same
+// lengths and counts.  The context is left configured with the winner.
+int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_iters, int32_t passes,
+                 spz_config *best, double *best_ms, int32_t *tried) {
+    if (!ctx || !boards || n <= 0 || !best) return SPZ_E_ARG;
+    if (passes <= 0) passes = 1;
+    std::vector<spz_result> ref((size_t)n), cur((size_t)n);
+    std::vector<uint64_t> ic_ref((size_t)n * max_iters), ic((size_t)n * max_iters);
+    spz_config base = ctx->cfg;
+    const int T = base.frontier_target;
+    const int wgs[3] = {64, 128, 256};
+    const int tgts[5] = {std::max(1, T / 16), std::max(1, T / 4), T, T * 2, T * 4};
+    const int deps_w4[4] = {48, 64, 80, 112}, deps_w5[4] = {64, 80, 96, 128};
+    const int *deps = base.width == 4 ? deps_w4 : deps_w5;
+    const int buds[3] = {256, 1024, 4096};
+    auto run = [&](const spz_config &c, double *ms, bool check) -> int {
+        int rc = spz_reconfigure(ctx, &c);
+        if (rc) return rc;
+        auto t = std::chrono::steady_clock::now();
+        rc = spz_solve_batch(ctx, boards, n, max_iters, check ? cur.data() : ref.data(),
+                             check ? ic.data() : ic_ref.data());
+        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+        if (rc) return rc;
+        if (check) {
+            for (int i = 0; i < n; ++i)
+                if (cur[i].opt_len != ref[i].opt_len || cur[i].total_nodes != ref[i].total_nodes)
+                    return fail(ctx, SPZ_E_STALL, "autotune: candidate changed the result of board %d", i);
+            if (memcmp(ic.data(), ic_ref.data(), ic.size() * 8) != 0)
+                return fail(ctx, SPZ_E_STALL, "autotune: candidate changed a per-iteration count");
+        }
+        return SPZ_OK;
+    };
+    spz_config bc = base;
+    if (bc.round_budget <= 0) bc.round_budget = 1024;
+    double bms = 0;
+    int count = 0;
+    int rc = run(bc, &bms, false); // reference result (also warms the buffers)
+    if (rc) return rc;
+    rc = run(bc, &bms, true);
+    if (rc) return rc;
+    count = 2;
+    for (int pass = 0; pass < passes; ++pass) {
+        for (int dim = 0; dim < 4; ++dim) {
+            int nv = dim == 0 ? 3 : dim == 1 ? 5 : dim == 2 ? 4 : 3;
+            spz_config winner = bc;
+            for (int v = 0; v < nv; ++v) {
+                spz_config c = bc;
+                if (dim == 0) c.workgroup_size = wgs[v];
+                if (dim == 1) c.frontier_target = tgts[v];
+                if (dim == 2) c.lds_stack_depth = deps[v];
+                if (dim == 3) c.round_budget = buds[v];
+                if (memcmp(&c, &bc, sizeof c) == 0) continue;
+                c.lanes = 0;
+                double ms = 0;
+                rc = run(c, &ms, true);
+                ++count;
+                if (rc) return rc;
+                if (ms < bms) {
+                    bms = ms;
+                    winner = c;
+                }
+            }
+            bc = winner;
+        }
+    }
+    rc = spz_reconfigure(ctx, &bc);
+    if (rc) return rc;
+    *best = ctx->cfg;
+    if (best_ms) *best_ms = bms;
+    if (tried) *tried = count;
     return SPZ_OK;
 }
 
@@ -594,11 +699,14 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         for (auto &x : ev) HIPCHK(hipEventCreate(&x));
     double round_ms = 0;
     long timed = 0;
-    unsigned *h_ctl = nullptr;
-    HIPCHK(hipHostMalloc((void **)&h_ctl, 64 * 4, hipHostMallocDefault));
+    unsigned *h_ctl = nullptr; // pinned: 64 control words, then the 3 queue counters
+    HIPCHK(hipHostMalloc((void **)&h_ctl, 64 * 4 + 3 * 8, hipHostMallocDefault));
+    unsigned long long *h_qc = (unsigned long long *)(h_ctl + 64);
     int rc = SPZ_OK;
     long rounds = 0;
-    for (;;) {
+
+    // one poll period: per_poll rounds, then the control words to the host
+    auto enqueue_period = [&]() -> hipError_t {
         hipError_t e = hipSuccess;
         for (int k = 0; k < per_poll && e == hipSuccess; ++k) {
             if (timing) e = hipEventRecord(ev[2 * k], s);
@@ -607,8 +715,34 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
             if (e == hipSuccess) e = launch_restart(p, s);
             if (e == hipSuccess) e = launch_control(p, s);
         }
-        rounds += per_poll;
         if (e == hipSuccess) e = hipMemcpyAsync(h_ctl, ctx->ctl.p, 64 * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(h_qc, ctx->q_ctl.p, 3 * 8, hipMemcpyDeviceToHost, s);
+        return e;
+    };
+    // hipGraph: the period is captured once and replayed; the kernels read
+    // all their state from device memory, so replays need no new arguments
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool use_graph = ctx->cfg.use_graph != 0;
+    if (use_graph) {
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+        hipError_t e2 = e == hipSuccess ? enqueue_period() : e;
+        hipError_t e3 = e == hipSuccess ? hipStreamEndCapture(s, &graph) : e;
+        if (e2 == hipSuccess && e3 == hipSuccess) e3 = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+            // capture unsupported here: run the same sequence eagerly
+            if (gexec) hipGraphExecDestroy(gexec);
+            if (graph) hipGraphDestroy(graph);
+            gexec = nullptr;
+            graph = nullptr;
+            use_graph = false;
+            (void)hipGetLastError();
+        }
+    }
+    ctx->stats[15] = use_graph ? 1.0 : 0.0;
+    for (;;) {
+        hipError_t e = use_graph ? hipGraphLaunch(gexec, s) : enqueue_period();
+        rounds += per_poll;
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (timing && e == hipSuccess) {
             for (int k = 0; k < per_poll && e == hipSuccess; ++k) {
@@ -623,13 +757,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
             break;
         }
         if (h_ctl[1] == 0) break;
-        unsigned long long qc[3];
-        e = hipMemcpy(qc, ctx->q_ctl.p, sizeof qc, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
-            rc = fail(ctx, SPZ_E_HIP, "search rounds: %s", hipGetErrorString(e));
-            break;
-        }
-        stalled = (qc[2] == qc[0] && h_ctl[2] == 0 && h_ctl[0] == 0) ? stalled + 1 : 0;
+        stalled = (h_qc[2] == h_qc[0] && h_ctl[2] == 0 && h_ctl[0] == 0) ? stalled + 1 : 0;
         if (stalled > 8) {
             rc = fail(ctx, SPZ_E_STALL, "search stalled with %u active instances", h_ctl[1]);
             break;
@@ -640,6 +768,8 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
             break;
         }
     }
+    if (gexec) hipGraphExecDestroy(gexec);
+    if (graph) hipGraphDestroy(graph);
     if (timing)
         for (auto &x : ev) hipEventDestroy(x);
     if (rc != SPZ_OK) {

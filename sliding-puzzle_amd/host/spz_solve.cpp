@@ -1,0 +1,190 @@
+// spz_solve -- command-line front end of libspz.
+//
+// Takes over from the reference's solver processes: read problem files
+// (N*N whitespace-separated integers, 0 = blank; src/24puzzle.cc:72-87,
+// src_horie/idas_cpu.c:336-360), solve, and print one line per file in the
+// reference's format "<path> <len> <nodes>" (cumulative all-solutions count,
+// src/24puzzle_expand.cc:154).  --iters adds the per-iteration lines of
+// src_horie/idas_cpu_expand.c:288-304 ("f_limit=%3d, expanded nodes = %lld"),
+// --csv writes per-instance seconds like the reference's result/*.csv files.
+#include "spz.h"
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+static void usage() {
+    std::fprintf(stderr,
+                 "usage: spz_solve [options] FILE...\n"
+                 "  -H manhattan|pdb       heuristic (default manhattan)\n"
+                 "  -m all|first           all solutions of the last iteration (default) or stop at a goal\n"
+                 "  --batch                solve all files in one batch (default: one per launch)\n"
+                 "  --iters                print per-iteration counts\n"
+                 "  --csv FILE             write per-instance seconds\n"
+                 "  --pdb-cache FILE       read the PDB tables from FILE, or write them there\n"
+                 "  --export-tab DIR       write pat24.1256712.tab and pat24.34891314.tab\n"
+                 "  --lb uniform|prevload  dispatch order of frontier nodes\n"
+                 "  --frontier N --wg N --lds-depth N --budget N --max-iters N\n"
+                 "  --autotune             tune the launch configuration on the files first\n");
+}
+
+static bool read_board(const std::string &path, std::vector<uint8_t> &b, int &width) {
+    std::ifstream f(path);
+    if (!f) return false;
+    std::vector<int> v;
+    int x;
+    while (f >> x) v.push_back(x);
+    int w = v.size() >= 25 ? 5 : (v.size() >= 16 ? 4 : 0);
+    if (w == 0) return false;
+    if (width == 0) width = w;
+    if (w != width) return false;
+    for (int i = 0; i < width * width; ++i) b.push_back((uint8_t)v[i]);
+    return true;
+}
+
+int main(int argc, char **argv) {
+    spz_config cfg;
+    int heur = SPZ_H_MANHATTAN, mode = SPZ_MODE_ALL, lb = SPZ_LB_UNIFORM;
+    int frontier = 2048, wg = 256, depth = 0, budget = 0, max_iters = 64;
+    bool batch = false, iters = false, autotune = false;
+    std::string csv, cache, tabdir;
+    std::vector<std::string> files;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto next = [&]() -> const char * {
+            if (i + 1 >= argc) {
+                usage();
+                std::exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "-H") {
+            std::string v = next();
+            heur = v == "pdb" ? SPZ_H_PDB666 : SPZ_H_MANHATTAN;
+        } else if (a == "-m") {
+            mode = std::string(next()) == "first" ? SPZ_MODE_FIRST : SPZ_MODE_ALL;
+        } else if (a == "--batch") {
+            batch = true;
+        } else if (a == "--iters") {
+            iters = true;
+        } else if (a == "--csv") {
+            csv = next();
+        } else if (a == "--pdb-cache") {
+            cache = next();
+        } else if (a == "--export-tab") {
+            tabdir = next();
+        } else if (a == "--lb") {
+            std::string v = next();
+            lb = v == "prevload" ? SPZ_LB_PREVLOAD : SPZ_LB_UNIFORM;
+        } else if (a == "--frontier") {
+            frontier = std::atoi(next());
+        } else if (a == "--wg") {
+            wg = std::atoi(next());
+        } else if (a == "--lds-depth") {
+            depth = std::atoi(next());
+        } else if (a == "--budget") {
+            budget = std::atoi(next());
+        } else if (a == "--max-iters") {
+            max_iters = std::atoi(next());
+        } else if (a == "--autotune") {
+            autotune = true;
+        } else if (a == "-h" || a == "--help") {
+            usage();
+            return 0;
+        } else if (!a.empty() && a[0] == '-') {
+            usage();
+            return 2;
+        } else {
+            files.push_back(a);
+        }
+    }
+    if (files.empty() && tabdir.empty()) {
+        usage();
+        return 2;
+    }
+    int width = heur == SPZ_H_PDB666 ? 5 : 0;
+    std::vector<uint8_t> boards;
+    for (auto &f : files)
+        if (!read_board(f, boards, width)) {
+            std::fprintf(stderr, "spz_solve: cannot read a %s board from %s\n", width ? "matching" : "", f.c_str());
+            return 1;
+        }
+    if (width == 0) width = 5;
+    spz_default_config(&cfg, width, heur);
+    cfg.mode = mode;
+    cfg.frontier_target = frontier;
+    cfg.workgroup_size = wg;
+    cfg.lds_stack_depth = depth;
+    cfg.round_budget = budget;
+    cfg.lb_policy = lb;
+    spz_ctx *ctx = nullptr;
+    int rc = spz_create(&ctx, &cfg);
+    if (rc != SPZ_OK) {
+        std::fprintf(stderr, "spz_create: %s\n", ctx ? spz_last_error(ctx) : "bad configuration");
+        spz_destroy(ctx);
+        return 1;
+    }
+    if (heur == SPZ_H_PDB666 || !tabdir.empty()) {
+        rc = spz_build_pdb(ctx, cache.empty() ? nullptr : cache.c_str());
+        if (rc == SPZ_OK && !tabdir.empty()) {
+            std::string a = tabdir + "/pat24.1256712.tab", b = tabdir + "/pat24.34891314.tab";
+            rc = spz_export_pdb_tab(ctx, a.c_str(), b.c_str());
+        }
+        if (rc != SPZ_OK) {
+            std::fprintf(stderr, "spz_solve: %s\n", spz_last_error(ctx));
+            spz_destroy(ctx);
+            return 1;
+        }
+    }
+    const int n = (int)files.size(), n2 = width * width;
+    if (autotune && n > 0) {
+        spz_config best;
+        double ms = 0;
+        int tried = 0;
+        rc = spz_autotune(ctx, boards.data(), n, max_iters, 1, &best, &ms, &tried);
+        if (rc != SPZ_OK) {
+            std::fprintf(stderr, "spz_solve: autotune: %s\n", spz_last_error(ctx));
+            spz_destroy(ctx);
+            return 1;
+        }
+        std::fprintf(stderr, "autotune: wg=%d frontier=%d lds_depth=%d budget=%d (%.1f ms, %d candidates)\n",
+                     best.workgroup_size, best.frontier_target, best.lds_stack_depth, best.round_budget, ms, tried);
+    }
+    std::vector<spz_result> res((size_t)n);
+    std::vector<uint64_t> ic((size_t)n * max_iters);
+    std::vector<double> secs((size_t)n);
+    if (batch) {
+        auto t0 = std::chrono::steady_clock::now();
+        rc = spz_solve_batch(ctx, boards.data(), n, max_iters, res.data(), ic.data());
+        double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (auto &x : secs) x = s / n;
+    } else {
+        for (int i = 0; i < n && rc == SPZ_OK; ++i) {
+            auto t0 = std::chrono::steady_clock::now();
+            rc = spz_solve_batch(ctx, boards.data() + (size_t)i * n2, 1, max_iters, &res[i],
+                                 ic.data() + (size_t)i * max_iters);
+            secs[i] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+    }
+    if (rc != SPZ_OK) {
+        std::fprintf(stderr, "spz_solve: %s\n", spz_last_error(ctx));
+        spz_destroy(ctx);
+        return 1;
+    }
+    FILE *cf = csv.empty() ? nullptr : std::fopen(csv.c_str(), "w");
+    for (int i = 0; i < n; ++i) {
+        if (iters)
+            for (int k = 0; k < res[i].n_iters; ++k)
+                std::printf("f_limit=%3d, expanded nodes = %llu\n", res[i].h0 + 2 * k,
+                            (unsigned long long)ic[(size_t)i * max_iters + k]);
+        std::printf("%s %d %llu\n", files[i].c_str(), res[i].opt_len, (unsigned long long)res[i].total_nodes);
+        if (cf) std::fprintf(cf, "%f\n", secs[i]);
+    }
+    if (cf) std::fclose(cf);
+    spz_destroy(ctx);
+    return 0;
+}

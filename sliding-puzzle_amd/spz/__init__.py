@@ -45,7 +45,8 @@ class Config(ctypes.Structure):
                 ("lanes", ctypes.c_int32), ("round_budget", ctypes.c_int32),
                 ("lds_stack_depth", ctypes.c_int32),
                 ("lb_policy", ctypes.c_int32), ("device", ctypes.c_int32),
-                ("time_kernels", ctypes.c_int32), ("log_samples", ctypes.c_int32),
+                ("time_kernels", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("log_samples", ctypes.c_int32),
                 ("queue_items", ctypes.c_int64)]
 
 
@@ -74,7 +75,8 @@ EXPORTS = ["spz_default_config", "spz_create", "spz_build_pdb", "spz_export_pdb_
            "spz_last_error", "spz_destroy", "spz_pdb_geometry", "spz_gen_walk",
            "spz_gen_random", "spz_gather_probe", "spz_mlp_param_count", "spz_mlp_init",
            "spz_mlp_set_params", "spz_mlp_get_params", "spz_mlp_forward", "spz_mlp_grad",
-           "spz_mlp_train", "spz_get_samples"]
+           "spz_mlp_train", "spz_get_samples", "spz_reconfigure", "spz_get_config",
+           "spz_autotune"]
 
 
 def lib() -> ctypes.CDLL:
@@ -114,6 +116,12 @@ def lib() -> ctypes.CDLL:
     L.spz_mlp_train.argtypes = [vp, u8p, i32p, f32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                 ctypes.c_float, ctypes.c_uint64, f32p]
     L.spz_get_samples.argtypes = [vp, u8p, i32p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int32, i32p]
+    L.spz_reconfigure.argtypes = [vp, ctypes.POINTER(Config)]
+    L.spz_get_config.argtypes = [vp, ctypes.POINTER(Config)]
+    L.spz_get_config.restype = None
+    L.spz_autotune.argtypes = [vp, u8p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                               ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_int32)]
     L.spz_pdb_geometry.argtypes = [u8p, u8p]
     L.spz_pdb_geometry.restype = None
     L.spz_gen_walk.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, u8p]
@@ -173,7 +181,8 @@ class Solver:
                  frontier_target: int = 2048, max_split_depth: int = 24,
                  workgroup_size: int = 256, lanes: int = 0, round_budget: int = 0,
                  lb_policy: int = LB_UNIFORM, device: int = 0, queue_items: int = 0,
-                 time_kernels: bool = False, lds_stack_depth: int = 0, log_samples: int = 0):
+                 time_kernels: bool = False, lds_stack_depth: int = 0, log_samples: int = 0,
+                 use_graph: bool = True):
         L = lib()
         cfg = Config()
         L.spz_default_config(ctypes.byref(cfg), width, heuristic)
@@ -189,6 +198,7 @@ class Solver:
         cfg.time_kernels = 1 if time_kernels else 0
         cfg.lds_stack_depth = lds_stack_depth
         cfg.log_samples = log_samples
+        cfg.use_graph = 1 if use_graph else 0
         self.cfg = cfg
         self.width = width
         self._ctx = ctypes.c_void_p()
@@ -312,6 +322,27 @@ class Solver:
         k = n.value
         return b[:k], t[:k], ld[:k]
 
+    def config(self) -> dict:
+        c = Config()
+        lib().spz_get_config(self._ctx, ctypes.byref(c))
+        return {f: getattr(c, f) for f, _ in Config._fields_}
+
+    def reconfigure(self, **kw):
+        c = Config()
+        lib().spz_get_config(self._ctx, ctypes.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        self._check(lib().spz_reconfigure(self._ctx, ctypes.byref(c)), "spz_reconfigure")
+
+    def autotune(self, boards, max_iters: int = 64, passes: int = 1):
+        """Tunes this context in place on the sample boards; returns
+        (best config dict, best ms, candidates tried)."""
+        b = np.ascontiguousarray(np.asarray(boards, dtype=np.uint8).reshape(-1, self.width ** 2))
+        best, ms, n = Config(), ctypes.c_double(), ctypes.c_int32()
+        self._check(lib().spz_autotune(self._ctx, _u8p(b), len(b), max_iters, passes, ctypes.byref(best),
+                                       ctypes.byref(ms), ctypes.byref(n)), "spz_autotune")
+        return {f: getattr(best, f) for f, _ in Config._fields_}, ms.value, n.value
+
     def set_lb(self, policy: int):
         self._check(lib().spz_set_lb(self._ctx, policy), "spz_set_lb")
 
@@ -347,4 +378,4 @@ class Solver:
                 "pdb_levels0": s[4], "pdb_levels1": s[5], "pdb_build_ms": s[6],
                 "overflow_items": s[7], "round_kernel_ms": s[8], "round_launches": s[9],
                 "split_ms": s[10], "instances": s[11], "nodes": s[12],
-                "setup_ms": s[13], "rounds_wall_ms": s[14]}
+                "setup_ms": s[13], "rounds_wall_ms": s[14], "graph": s[15]}

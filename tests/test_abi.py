@@ -80,3 +80,31 @@ def test_default_config_values():
     spz.lib().spz_default_config(ctypes.byref(cfg), 5, spz.H_PDB666)
     assert (cfg.width, cfg.heuristic, cfg.mode, cfg.frontier_target) == (5, 1, 0, 2048)
     assert cfg.workgroup_size == 256 and cfg.time_kernels == 0
+
+
+def test_config_layout_matches_c(tmp_path):
+    """The ctypes mirror of spz_config has the C layout (offsets, size)."""
+    import ctypes
+    fields = [f for f, _ in spz.Config._fields_]
+    src = '#include <stdio.h>\n#include <stddef.h>\n#include "spz.h"\nint main(void){\n'
+    src += 'printf("%zu\\n", sizeof(spz_config));\n'
+    for f in fields:
+        src += f'printf("%zu\\n", offsetof(spz_config, {f}));\n'
+    src += "return 0;}\n"
+    c = tmp_path / "layout.c"
+    c.write_text(src)
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(c), "-o", str(exe)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(spz.Config)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(spz.Config, f).offset == off, f
+
+
+def test_cli_usage():
+    exe = os.path.join(REPO, "sliding-puzzle_amd", "bin", "spz_solve")
+    r = subprocess.run([exe, "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "usage: spz_solve" in r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 2

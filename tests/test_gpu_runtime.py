@@ -1,0 +1,84 @@
+"""hipGraph replay, the autotuner and the command-line front end, on the GPU."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import spz
+
+pytestmark = pytest.mark.gpu
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_graph_replay_matches_eager(golden):
+    """The rounds replayed from a captured hipGraph give the eager launches'
+    results, on several solves in a row (each solve replays its graph many
+    times with host synchronisation between replays)."""
+    fx = golden("iterations_korf100.json")["instances"][:10]
+    boards = [r["board"] for r in fx]
+    with spz.Solver(width=4, use_graph=False) as s:
+        eager = [(r.opt_len, r.counts) for r in s.solve(boards)]
+        assert s.stats()["graph"] == 0
+    with spz.Solver(width=4, use_graph=True) as s:
+        for rep in range(3):
+            say("graph solve", rep)
+            got = [(r.opt_len, r.counts) for r in s.solve(boards)]
+            assert s.stats()["graph"] == 1
+            assert got == eager
+            for rec, (ln, cnt) in zip(fx, got):
+                assert ln == rec["opt"]
+                assert all(b is None or a == b for a, b in zip(cnt, rec["counts"]))
+
+
+def test_autotuner_keeps_results(golden):
+    fx = golden("iterations_c1_walk40.json")["instances"][:32]
+    boards = [r["board"] for r in fx]
+    with spz.Solver(width=4) as s:
+        say("autotune")
+        best, ms, tried = s.autotune(boards)
+        say("best", best, ms, tried)
+        assert tried >= 10 and ms > 0
+        assert best["workgroup_size"] in (64, 128, 256)
+        assert s.config()["workgroup_size"] == best["workgroup_size"]
+        res = s.solve(boards)
+    for rec, r in zip(fx, res):
+        assert r.opt_len == rec["opt"]
+        assert all(b is None or a == b for a, b in zip(r.counts, rec["counts"]))
+
+
+def test_cli_reference_format(tmp_path, golden, ref_results):
+    """spz_solve prints "<path> <len> <nodes>" and, with --iters, the
+    reference's per-iteration lines."""
+    fx = golden("iterations_korf100.json")["instances"][:5]
+    paths = []
+    for rec in fx:
+        p = tmp_path / f"prob{rec['idx']:03d}"
+        p.write_text(" ".join(map(str, rec["board"])) + " \n")
+        paths.append(str(p))
+    exe = os.path.join(REPO, "sliding-puzzle_amd", "bin", "spz_solve")
+    csv = tmp_path / "times.csv"
+    r = subprocess.run([exe, "--iters", "--csv", str(csv), *paths], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    res_lines = [l for l in lines if not l.startswith("f_limit")]
+    it_lines = [l for l in lines if l.startswith("f_limit")]
+    assert len(res_lines) == len(fx)
+    for rec, path, line in zip(fx, paths, res_lines):
+        toks = line.split()
+        assert toks[0] == path and int(toks[1]) == rec["opt"]
+        assert int(toks[1]) == ref_results["result_15puzzle"]["rows"][rec["idx"]]["len"]
+        assert int(toks[2]) == sum(c for c in rec["counts"] if c is not None) or None in rec["counts"]
+    assert len(it_lines) == sum(len(rec["thresholds"]) for rec in fx)
+    assert it_lines[0].startswith("f_limit=") and "expanded nodes = " in it_lines[0]
+    assert len(csv.read_text().split()) == len(fx)
+    r2 = subprocess.run([exe, "--batch", *paths], capture_output=True, text=True, timeout=120)
+    assert r2.returncode == 0 and r2.stdout.splitlines() == res_lines
