@@ -226,7 +226,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this solver runs on one GPU; --gpus above 1 is not supported "
-                                   "(instances shard trivially across GPUs, see DESIGN.md)"}))
+                                   "(instance batches shard across GPUs with no exchange; not built here)"}))
         sys.exit(2)
 
     if args.copy_probe:
