@@ -72,7 +72,8 @@ typedef struct spz_config {
     int32_t lanes;           /* DFS lanes in flight (0 = fill the device) */
     int32_t round_budget;    /* DFS steps per lane per round (0 = auto) */
     int32_t lds_stack_depth; /* DFS stack levels per lane kept in LDS (0 =
-                                80 for width 4, 96 for width 5; at most 160);
+                                80 for width 4, 96 for width 5 Manhattan, 64
+                                for the PDB search; at most 160);
                                 deeper subtrees spill to the work queue */
     int32_t lb_policy;       /* SPZ_LB_* */
     int32_t device;          /* HIP device ordinal */
@@ -178,14 +179,16 @@ int spz_mlp_train(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const
  * boards == NULL, the number available. */
 int spz_get_samples(spz_ctx *ctx, uint8_t *boards, int32_t *thr, uint64_t *loads, int32_t max, int32_t *n_out);
 
-/* Statistics of the last solve (stats must hold 16 doubles):
+/* Statistics of the last solve (stats must hold 24 doubles):
  *  0 rounds run            1 solve wall ms        2 items donated
  *  3 lanes                 4,5 PDB BFS levels     6 PDB build ms
  *  7 stack-overflow items  8 round-kernel ms (sum, when time_kernels)
  *  9 round launches timed  10 split ms            11 instances
  *  12 nodes (all instances, all iterations)
  *  13 setup ms (upload, root split, first restart)  14 rounds wall ms
- *  15 1 if the rounds ran from a hipGraph */
+ *  15 1 if the rounds ran from a hipGraph
+ *  16-19 with env SPZ_PROFILE=1: lane-steps idle, expanding, only popping,
+ *        and wave-steps (x64 = lane-steps) of the search rounds */
 int spz_last_stats(spz_ctx *ctx, double *stats);
 
 const char *spz_last_error(spz_ctx *ctx);

@@ -27,6 +27,8 @@ template <> struct Geo<5> {
     static constexpr int MAXD = 160;
     static constexpr int DEFD = 96;
 };
+// default LDS stack depth of the PDB search (its entries are two bytes)
+constexpr int kDefDepthPdb = 64;
 
 // goal row / column of tile t (tile t belongs at cell t)
 template <int W> __host__ __device__ __forceinline__ int trow(int t) {
@@ -286,6 +288,49 @@ __device__ __forceinline__ void do_move_known(Pos<W, 1> &s, int d, const PdbCtx 
     s.br = nr;
     s.bc = nc;
 }
+
+// values, before the move, of the two lookups containing the tile that a
+// blank move in direction d slides (packed regular | reflected << 8)
+template <int W>
+__device__ __forceinline__ uint32_t pdb_moved_vals(const Pos<W, 1> &s, int d, const PdbCtx &P) {
+    int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
+    int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
+    int t = s.bd.tile(nr * W + nc);
+    const PdbGeo &G = *P.geo;
+    int kr = G.reg_k[t], kf = G.ref_k[t] - 4;
+    return (uint32_t)byte_of(s.hs.v0, kr) | ((uint32_t)byte_of(s.hs.v1, kf) << 8);
+}
+
+// One DFS stack level: the move d taken from it, the passing children R
+// still to visit (always moves after d), and for the PDB search the values
+// the move overwrote, so that undoing it needs no table reads.
+template <int HEUR> struct StackEntry;
+template <> struct StackEntry<0> {
+    using T = uint8_t;
+    static __device__ __forceinline__ T make(int d, uint32_t R, uint32_t) { return (T)((R << 2) | (uint32_t)d); }
+    static __device__ __forceinline__ int dir(uint32_t e) { return (int)(e & 3u); }
+    static __device__ __forceinline__ uint32_t rest(uint32_t e) { return e >> 2; }
+    static __device__ __forceinline__ uint32_t vals(uint32_t) { return 0; }
+};
+template <> struct StackEntry<1> {
+    using T = uint16_t;
+    // bits 0-3: code of (d, R) = base(d) + (R >> (d+1)), base = 0, 8, 12, 14
+    // bits 4-9 / 10-15: regular / reflected value before the move (< 64)
+    static __device__ __forceinline__ uint32_t base(int d) { return d == 0 ? 0u : d == 1 ? 8u : d == 2 ? 12u : 14u; }
+    static __device__ __forceinline__ T make(int d, uint32_t R, uint32_t v) {
+        uint32_t code = base(d) + (R >> (d + 1));
+        return (T)(code | ((v & 63u) << 4) | (((v >> 8) & 63u) << 10));
+    }
+    static __device__ __forceinline__ int dir(uint32_t e) {
+        uint32_t c = e & 15u;
+        return c >= 14u ? 3 : (c >= 12u ? 2 : (c >= 8u ? 1 : 0));
+    }
+    static __device__ __forceinline__ uint32_t rest(uint32_t e) {
+        int d = dir(e);
+        return ((e & 15u) - base(d)) << (d + 1);
+    }
+    static __device__ __forceinline__ uint32_t vals(uint32_t e) { return ((e >> 4) & 63u) | (((e >> 10) & 63u) << 8); }
+};
 
 template <int W, int HEUR>
 __device__ __forceinline__ void init_heur(Pos<W, HEUR> &s, const PdbCtx &P, uint32_t meta) {

@@ -72,7 +72,8 @@ struct KParams {
     int *restart_list;
     unsigned int *ctl;           // 0 n_restart, 1 active, 2 busy, 3 error,
                                  // 4 donate_per_lane, 5 budget, 6 rounds,
-                                 // 7 donated, 8 overflow exports
+                                 // 7 donated, 8 overflow exports, 9/10 big/small
+                                 // budget, 12 busy at last round end
 
     // top tree of the root split (per instance)
     const Item *frontier;        // [n_inst][capl]
@@ -93,6 +94,7 @@ struct KParams {
     // lane contexts, structure of arrays over n_lanes lanes
     int n_lanes;
     int maxd;          // LDS stack levels per lane in use
+    unsigned long long *prof; // null, or [4]: lane-steps idle, expanding, popping only, wave-steps
     uint32_t *c_st;    // state | sp<<8
     uint32_t *c_rows;  // [5][n_lanes]
     uint32_t *c_root;  // [4][n_lanes] packed item root

@@ -21,6 +21,9 @@
 
 namespace spz {
 
+constexpr int kPops = 4; // stack levels a lane may pop in one step
+
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t lo = __builtin_amdgcn_mbcnt_lo(~0u, 0u);
     uint32_t n = __builtin_amdgcn_mbcnt_hi(~0u, lo);
@@ -98,7 +101,10 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
     const int nw = blockDim.x >> 6;
     const int gl = blockIdx.x * blockDim.x + threadIdx.x;
     const int NL = p.n_lanes;
-    uint8_t *stk = lds + wave * 64 + lane; // level k at stk[k * sstride]
+    using SE = StackEntry<HEUR>;
+    using ST = typename SE::T;
+    ST *stk = (ST *)lds + wave * 64 + lane; // level k at stk[k * sstride]
+    ST *cstk = (ST *)p.c_stack;
     const int sstride = nw * 64;
 
     PdbCtx P{&geo_s, p.pdb0, p.pdb1};
@@ -159,67 +165,82 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
             s.hs.B = (int)((v1 & 0xff) + ((v1 >> 8) & 0xff) + ((v1 >> 16) & 0xff) + (v1 >> 24));
         }
         L = p.L[inst];
-        for (int k = 0; k < sp; ++k) stk[k * sstride] = p.c_stack[(size_t)k * NL + gl];
+        for (int k = 0; k < sp; ++k) stk[k * sstride] = cstk[(size_t)k * NL + gl];
     } else {
         st = ST_IDLE;
     }
 
-    for (int step = 0; step < budget; ++step) {
-        // ---- take items for idle lanes (one atomic per wave) ----
-        uint64_t need = __ballot(st == ST_IDLE);
-        if (need) {
-            int n = __popcll(need);
-            int leader = __ffsll((long long)need) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&p.q_ctl[0], (unsigned long long)n);
-            base = shfl_u64(base, leader);
-            if (st == ST_IDLE) {
-                unsigned long long my = base + __popcll(need & lanemask_lt());
-                if (my < limit) {
-                    const Item it = p.q[my & p.qmask];
-                    pos_from_item<W, HEUR>(s, it.b, it.meta, P);
+    auto start_item = [&](const Item &it) {
+        pos_from_item<W, HEUR>(s, it.b, it.meta, P);
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) root[k] = it.b[k];
-                    rmeta = it.meta;
-                    inst = (int)it.inst;
-                    origin = it.x1;
-                    g = meta_g(it.meta);
-                    prev = meta_prev(it.meta);
-                    L = p.L[inst];
-                    sp = 0;
-                    cnt = 0;
-                    st = ST_FRESH;
-                    if (mode == 1 && p.found[inst] != kNoGoal) st = ST_POP; // first-solution: abandon
-                } else {
-                    st = ST_DRAINED;
+        for (int k = 0; k < 4; ++k) root[k] = it.b[k];
+        rmeta = it.meta;
+        inst = (int)it.inst;
+        origin = it.x1;
+        g = meta_g(it.meta);
+        prev = meta_prev(it.meta);
+        L = p.L[inst];
+        sp = 0;
+        cnt = 0;
+        st = ST_FRESH;
+        if (mode == 1 && p.found[inst] != kNoGoal) st = ST_POP; // first-solution: abandon
+    };
+
+    uint32_t pf_idle = 0, pf_exp = 0, pf_pop = 0, pf_steps = 0;
+    for (int step = 0; step < budget; ++step) {
+        // ---- idle lanes take queue items (one atomic per wave) ----
+        {
+            uint64_t need = __ballot(st == ST_IDLE);
+            if (need) {
+                int n = __popcll(need);
+                int leader = __ffsll((long long)need) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(&p.q_ctl[0], (unsigned long long)n);
+                base = shfl_u64(base, leader);
+                if (st == ST_IDLE) {
+                    unsigned long long my = base + __popcll(need & lanemask_lt());
+                    if (my < limit)
+                        start_item(p.q[my & p.qmask]);
+                    else
+                        st = ST_DRAINED;
                 }
             }
         }
         if (!__ballot(st == ST_FRESH || st == ST_POP)) break;
 
-        // ---- pop one level ----
+        // ---- pop: undo up to kPops moves, then enter the next sibling ----
         bool finish = false;
-        if (st == ST_POP) {
+        int sib = -1;
+#pragma unroll 1
+        for (int pk = 0; pk < kPops && st == ST_POP; ++pk) {
             if (sp == 0) {
                 finish = true;
-            } else {
-                uint32_t e = stk[(sp - 1) * sstride];
-                int d = (int)(e & 3u);
-                uint32_t R = e >> 2;
-                do_move<W, HEUR>(s, 3 - d, P);
-                --g;
-                if (R) {
-                    int d2 = __builtin_ctz(R);
-                    R &= R - 1;
-                    stk[(sp - 1) * sstride] = (uint8_t)((R << 2) | (uint32_t)d2);
-                    do_move<W, HEUR>(s, d2, P);
-                    ++g;
-                    prev = d2;
-                    st = ST_FRESH;
-                } else {
-                    --sp;
-                }
+                break;
             }
+            uint32_t e = stk[(sp - 1) * sstride];
+            int d = SE::dir(e);
+            uint32_t R = SE::rest(e);
+            if constexpr (HEUR == 1)
+                do_move_known<W>(s, 3 - d, P, SE::vals(e));
+            else
+                do_move<W, HEUR>(s, 3 - d, P);
+            --g;
+            if (R) {
+                int d2 = __builtin_ctz(R);
+                R &= R - 1;
+                uint32_t ov = 0;
+                if constexpr (HEUR == 1) ov = pdb_moved_vals<W>(s, d2, P);
+                stk[(sp - 1) * sstride] = SE::make(d2, R, ov);
+                sib = d2;
+                st = ST_FRESH;
+            } else {
+                --sp;
+            }
+        }
+        if (sib >= 0) {
+            do_move<W, HEUR>(s, sib, P);
+            ++g;
+            prev = sib;
         }
         if (finish) {
             if (cnt) {
@@ -233,26 +254,34 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
             st = ST_IDLE;
         }
 
-        // ---- expand ----
+        if (p.prof) {
+            pf_steps += 1;
+            pf_idle += (st == ST_IDLE || st == ST_DRAINED) ? 1u : 0u;
+            pf_exp += (st == ST_FRESH) ? 1u : 0u;
+            pf_pop += (st == ST_POP) ? 1u : 0u;
+        }
+        // ---- expand: f-tests of the children ----
+        bool xp = false;
+        uint32_t pass = 0;
+        uint32_t cvals = 0, cvals2 = 0; // PDB values of children 0,1 / 2,3 (16 bits each)
         if (st == ST_FRESH) {
             const int h = s.hs.h;
-            bool expand = true;
+            xp = true;
             if (h == 0) {
                 atomicMin(&p.found[inst], g);
                 if (mode == 1) {
                     st = ST_POP;
-                    expand = false;
+                    xp = false;
                 }
             }
-            if (expand) {
+            if (xp) {
                 const int br = s.br, bc = s.bc;
                 uint32_t legal = (br > 0 ? 1u : 0u) | (bc < W - 1 ? 2u : 0u) | (bc > 0 ? 4u : 0u) |
                                  (br < W - 1 ? 8u : 0u);
                 uint32_t gen = legal & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
                 cnt += (uint32_t)__builtin_popcount(gen);
-                uint32_t pass = gen;
+                pass = gen;
                 const int bl = br * W + bc;
-                uint32_t cvals = 0, cvals2 = 0; // PDB values of children 0,1 / 2,3 (16 bits each)
                 if constexpr (HEUR == 0) {
                     // Manhattan is consistent: h changes by exactly +-1 per
                     // move and f keeps the threshold's parity, so with slack
@@ -291,41 +320,48 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                         }
                     }
                 }
-                if (pass) {
-                    if (sp < MAXD) {
-                        int d = __builtin_ctz(pass);
-                        uint32_t R = pass & (pass - 1);
-                        stk[sp * sstride] = (uint8_t)((R << 2) | (uint32_t)d);
-                        ++sp;
-                        if constexpr (HEUR == 1) {
-                            uint32_t w2 = (d >= 2) ? cvals2 : cvals;
-                            do_move_known<W>(s, d, P, (w2 >> (16 * (d & 1))) & 0xffffu);
-                        } else {
-                            do_move<W, HEUR>(s, d, P);
-                        }
-                        ++g;
-                        prev = d;
+            }
+        }
+
+        // ---- expand: descend into the first passing child ----
+        if (xp) {
+            if (pass) {
+                if (sp < MAXD) {
+                    int d = __builtin_ctz(pass);
+                    uint32_t R = pass & (pass - 1);
+                    uint32_t ov = 0;
+                    if constexpr (HEUR == 1) ov = pdb_moved_vals<W>(s, d, P);
+                    stk[sp * sstride] = SE::make(d, R, ov);
+                    ++sp;
+                    if constexpr (HEUR == 1) {
+                        uint32_t w2 = (d >= 2) ? cvals2 : cvals;
+                        do_move_known<W>(s, d, P, (w2 >> (16 * (d & 1))) & 0xffffu);
                     } else {
-                        // stack full: the passing children become queue items
-                        int k = __builtin_popcount(pass);
-                        atomicAdd(&p.pend[inst], (unsigned)k);
-                        unsigned long long pos = atomicAdd(&p.q_ctl[2], (unsigned long long)k);
-                        atomicAdd(&p.ctl[8], (unsigned)k);
-                        while (pass) {
-                            int d = __builtin_ctz(pass);
-                            pass &= pass - 1;
-                            Pos<W, HEUR> c = s;
-                            do_move<W, HEUR>(c, d, P);
-                            put_item<W, HEUR>(p, pos++, c, g + 1, d, inst, origin);
-                        }
-                        st = ST_POP;
+                        do_move<W, HEUR>(s, d, P);
                     }
+                    ++g;
+                    prev = d;
                 } else {
+                    // stack full: the passing children become queue items
+                    int k = __builtin_popcount(pass);
+                    atomicAdd(&p.pend[inst], (unsigned)k);
+                    unsigned long long pos = atomicAdd(&p.q_ctl[2], (unsigned long long)k);
+                    atomicAdd(&p.ctl[8], (unsigned)k);
+                    while (pass) {
+                        int d = __builtin_ctz(pass);
+                        pass &= pass - 1;
+                        Pos<W, HEUR> c = s;
+                        do_move<W, HEUR>(c, d, P);
+                        put_item<W, HEUR>(p, pos++, c, g + 1, d, inst, origin);
+                    }
                     st = ST_POP;
                 }
+            } else {
+                st = ST_POP;
             }
         }
     }
+
 
     // ---- round end: donate shallow open nodes, then save the context ----
     const bool busy = (st == ST_FRESH || st == ST_POP);
@@ -333,7 +369,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
     int want = 0;
     if (busy && per_lane > 0) {
         int avail = 0;
-        for (int k = 0; k < sp && avail < per_lane; ++k) avail += __builtin_popcount((uint32_t)stk[k * sstride] >> 2);
+        for (int k = 0; k < sp && avail < per_lane; ++k) avail += __builtin_popcount(SE::rest(stk[k * sstride]));
         want = avail < per_lane ? avail : per_lane;
     }
     if (__ballot(want > 0)) {
@@ -361,8 +397,8 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
             int emitted = 0;
             for (int k = 0; k < sp && emitted < want; ++k) {
                 uint32_t e = stk[k * sstride];
-                int d = (int)(e & 3u);
-                uint32_t R = e >> 2;
+                int d = SE::dir(e);
+                uint32_t R = SE::rest(e);
                 while (R && emitted < want) {
                     int d2 = __builtin_ctz(R);
                     R &= R - 1;
@@ -371,7 +407,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                     put_item<W, HEUR>(p, pos++, x, cg + 1, d2, inst, origin);
                     ++emitted;
                 }
-                stk[k * sstride] = (uint8_t)((R << 2) | (uint32_t)d);
+                stk[k * sstride] = SE::make(d, R, SE::vals(e));
                 if (emitted < want) {
                     do_move<W, HEUR>(c, d, P);
                     ++cg;
@@ -385,6 +421,20 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
         atomicAdd(&p.cnt[inst], (unsigned long long)cnt);
         atomicAdd(&p.load[(size_t)inst * p.capl + origin], (unsigned long long)cnt);
         cnt = 0;
+    }
+    if (p.prof) {
+        // wave sums of this round's lane-step states
+        for (int off = 32; off > 0; off >>= 1) {
+            pf_idle += __shfl_xor(pf_idle, off);
+            pf_exp += __shfl_xor(pf_exp, off);
+            pf_pop += __shfl_xor(pf_pop, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&p.prof[0], (unsigned long long)pf_idle);
+            atomicAdd(&p.prof[1], (unsigned long long)pf_exp);
+            atomicAdd(&p.prof[2], (unsigned long long)pf_pop);
+            atomicAdd(&p.prof[3], (unsigned long long)pf_steps);
+        }
     }
     uint64_t busy_mask = __ballot(busy);
     if (lane == 0 && busy_mask) atomicAdd(&p.ctl[2], (unsigned)__popcll(busy_mask));
@@ -413,7 +463,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
             p.c_pdb[8 * NL + gl] = s.hs.v0;
             p.c_pdb[9 * NL + gl] = s.hs.v1;
         }
-        for (int k = 0; k < sp; ++k) p.c_stack[(size_t)k * NL + gl] = stk[k * sstride];
+        for (int k = 0; k < sp; ++k) cstk[(size_t)k * NL + gl] = stk[k * sstride];
     } else {
         p.c_st[gl] = ST_IDLE;
     }
@@ -560,6 +610,7 @@ __global__ void k_control(KParams p) {
     unsigned long long qlen = tail - head;
     unsigned busy = p.ctl[2];
     p.ctl[2] = 0;
+    p.ctl[12] = busy; // lanes busy at the end of the last round, for the host
     p.ctl[0] = 0; // restart list consumed
     unsigned long long lanes = (unsigned long long)p.n_lanes;
     unsigned long long idle = lanes > busy ? lanes - busy : 0ull;
@@ -687,7 +738,7 @@ __global__ void k_scatter_roots(const Item *roots, Item *lvl, int capl, int n) {
 // host-side launchers
 template <int W, int HEUR>
 static hipError_t launch_round_t(const KParams &p, const PdbGeo *geo, int wg, hipStream_t s) {
-    size_t lds = (size_t)p.maxd * wg;
+    size_t lds = (size_t)p.maxd * wg * sizeof(typename StackEntry<HEUR>::T);
     hipLaunchKernelGGL((k_round<W, HEUR>), dim3(p.n_lanes / wg), dim3(wg), lds, s, p, geo);
     return hipGetLastError();
 }
@@ -698,7 +749,7 @@ hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo
     return launch_round_t<5, 1>(p, geo, wg, s);
 }
 
-size_t round_lds_bytes(int maxd, int wg) { return (size_t)maxd * wg; }
+size_t round_lds_bytes(int maxd, int wg, int heur) { return (size_t)maxd * wg * (heur == 1 ? 2 : 1); }
 
 const void *round_kernel_ptr(int width, int heur) {
     if (width == 4) return (const void *)&k_round<4, 0>;

@@ -372,10 +372,11 @@ class Solver:
         return res
 
     def stats(self) -> dict:
-        s = (ctypes.c_double * 16)()
+        s = (ctypes.c_double * 24)()
         self._check(lib().spz_last_stats(self._ctx, s), "spz_last_stats")
         return {"rounds": s[0], "solve_ms": s[1], "donated": s[2], "lanes": s[3],
                 "pdb_levels0": s[4], "pdb_levels1": s[5], "pdb_build_ms": s[6],
                 "overflow_items": s[7], "round_kernel_ms": s[8], "round_launches": s[9],
                 "split_ms": s[10], "instances": s[11], "nodes": s[12],
-                "setup_ms": s[13], "rounds_wall_ms": s[14], "graph": s[15]}
+                "setup_ms": s[13], "rounds_wall_ms": s[14], "graph": s[15],
+                "prof_idle": s[16], "prof_expand": s[17], "prof_pop": s[18], "prof_wave_steps": s[19]}
