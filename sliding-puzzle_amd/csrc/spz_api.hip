@@ -75,7 +75,7 @@ struct spz_ctx {
     PdbGeo geo_h;
     DBuf geo_d;
     DBuf pdb_compact[2]; // file-order tables
-    DBuf pdb_direct[2];  // 25^6 addressing used by the search
+    DBuf pdb_direct;     // both tables in 25^6 addressing, end to end (the search's)
     bool have_pdb = false;
 
     // lane contexts
@@ -361,8 +361,8 @@ const char *spz_last_error(spz_ctx *ctx) { return ctx ? ctx->err.c_str() : "null
 
 void spz_destroy(spz_ctx *ctx) {
     if (!ctx) return;
-    DBuf *all[] = {&ctx->geo_d,       &ctx->pdb_compact[0], &ctx->pdb_compact[1], &ctx->pdb_direct[0],
-                   &ctx->pdb_direct[1], &ctx->c_st,         &ctx->c_rows,         &ctx->c_root,
+    DBuf *all[] = {&ctx->geo_d,       &ctx->pdb_compact[0], &ctx->pdb_compact[1], &ctx->pdb_direct,
+                   &ctx->c_st,         &ctx->c_rows,         &ctx->c_root,
                    &ctx->c_meta,      &ctx->c_rmeta,        &ctx->c_inst,         &ctx->c_cnt,
                    &ctx->c_pdb,       &ctx->c_stack,        &ctx->q,              &ctx->q_ctl,
                    &ctx->ctl,         &ctx->roots,          &ctx->L,              &ctx->it,
@@ -394,10 +394,10 @@ int spz_set_lb(spz_ctx *ctx, int32_t lb_policy) {
 
 // ---------------------------------------------------------------- PDB ---
 static int upload_direct(spz_ctx *ctx) {
-    for (int t = 0; t < 2; ++t) {
-        HIPCHK(ctx->pdb_direct[t].ensure(kPdbDirect));
-        HIPCHK(pdb_spread(ctx->pdb_compact[t].as<uint8_t>(), ctx->pdb_direct[t].as<uint8_t>(), ctx->stream));
-    }
+    HIPCHK(ctx->pdb_direct.ensure(2 * kPdbDirect));
+    for (int t = 0; t < 2; ++t)
+        HIPCHK(pdb_spread(ctx->pdb_compact[t].as<uint8_t>(), ctx->pdb_direct.as<uint8_t>() + t * kPdbDirect,
+                          ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_pdb = true;
     return SPZ_OK;
@@ -530,8 +530,7 @@ static KParams base_params(spz_ctx *ctx, int n, int max_iters, int capl) {
     p.max_samples = ctx->cfg.log_samples > 0 && ctx->samples.p ? (unsigned)ctx->cfg.log_samples : 0u;
     p.c_pdb = ctx->c_pdb.as<uint32_t>();
     p.c_stack = ctx->c_stack.as<uint8_t>();
-    p.pdb0 = ctx->pdb_direct[0].as<uint8_t>();
-    p.pdb1 = ctx->pdb_direct[1].as<uint8_t>();
+    p.pdb = ctx->pdb_direct.as<uint8_t>();
     return p;
 }
 
@@ -847,11 +846,11 @@ int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, double *ms, double *gather
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
-    hipError_t e = launch_gather_probe(ctx->pdb_direct[0].as<uint8_t>(), ctx->pdb_direct[1].as<uint8_t>(), blocks,
+    hipError_t e = launch_gather_probe(ctx->pdb_direct.as<uint8_t>(), ctx->pdb_direct.as<uint8_t>() + kPdbDirect, blocks,
                                        per_thread, 12345, out.as<uint32_t>(), ctx->stream); // warm
     if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
     if (e == hipSuccess)
-        e = launch_gather_probe(ctx->pdb_direct[0].as<uint8_t>(), ctx->pdb_direct[1].as<uint8_t>(), blocks,
+        e = launch_gather_probe(ctx->pdb_direct.as<uint8_t>(), ctx->pdb_direct.as<uint8_t>() + kPdbDirect, blocks,
                                 per_thread, 777, out.as<uint32_t>(), ctx->stream);
     if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
     if (e == hipSuccess) e = hipEventSynchronize(b);

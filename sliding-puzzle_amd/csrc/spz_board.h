@@ -117,7 +117,13 @@ struct PdbGeo {
     uint8_t pad[2];
     uint32_t reg_mul[25]; // 25^(5-slot) of tile t in reg_k
     uint32_t ref_mul[25];
+    // index weight of tile t on cell c in its regular / reflected lookup,
+    // map_k(c) * 25^(5-slot) (< 2^28), with k (0..3) in bits 28-29: a move's
+    // index change is w[t][to] - w[t][from] (the k bits cancel)
+    uint32_t wr[25][25];
+    uint32_t wf[25][25];
 };
+constexpr uint32_t kWeightMask = 0x0fffffffu;
 
 // Build the geometry on the host from the cell symmetries (the reference's
 // rf / rot90 / rot180 tables, src/24puzzle_with_pdb.cc:33-46).
@@ -137,15 +143,16 @@ template <> struct HState<0> {
 template <> struct HState<1> {
     int h;
     int A, B;                   // regular / reflected sums, h = max(A, B)
-    uint32_t i0, i1, i2, i3;    // direct indices, regular lookups 0..3
-    uint32_t j0, j1, j2, j3;    // direct indices, reflected lookups 4..7
+    // direct indices into the two tables laid end to end (table 1 starts at
+    // kPdbDirect; lookups 0 and 4 read table 0, the others table 1)
+    uint32_t i0, i1, i2, i3;    // regular lookups 0..3
+    uint32_t j0, j1, j2, j3;    // reflected lookups 4..7
     uint32_t v0, v1;            // lookup values, bytes 0..3 = lookups 0..3 / 4..7
 };
 
 struct PdbCtx {
-    const PdbGeo *geo; // LDS copy
-    const uint8_t *t0;
-    const uint8_t *t1;
+    const PdbGeo *geo;  // LDS copy
+    const uint8_t *tab; // table 0 at 0, table 1 at kPdbDirect
 };
 
 __device__ __forceinline__ uint32_t sel4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, int k) {
@@ -164,12 +171,8 @@ __device__ __forceinline__ uint32_t set_byte(uint32_t w, int k, int v) {
     return (w & ~(0xffu << (8 * k))) | ((uint32_t)v << (8 * k));
 }
 
-// kr in 0..3 (regular lookups: table 0 for kr == 0), kf in 0..3 (reflected
-// lookups 4..7: table 0 for kf == 0)
-__device__ __forceinline__ int pdb_read_k(const PdbCtx &P, int k03, uint32_t idx) {
-    const uint8_t *t = (k03 == 0) ? P.t0 : P.t1;
-    return (int)t[idx];
-}
+// idx carries its table's offset, so every lookup is one load from one base
+__device__ __forceinline__ int pdb_read(const PdbCtx &P, uint32_t idx) { return (int)P.tab[idx]; }
 
 // Heuristic of the child reached by moving tile t from cell a to cell b,
 // without changing the state (f-tests at expansion); the two values read
@@ -177,13 +180,12 @@ __device__ __forceinline__ int pdb_read_k(const PdbCtx &P, int k03, uint32_t idx
 __device__ __forceinline__ int pdb_child_h(const PdbCtx &P, const HState<1> &s, int t, int a, int b,
                                            uint32_t &vals) {
     const PdbGeo &G = *P.geo;
-    int kr = G.reg_k[t], kf = G.ref_k[t] - 4;
-    uint32_t ir = sel4(s.i0, s.i1, s.i2, s.i3, kr) +
-                  (uint32_t)((int)G.map[kr][b] - (int)G.map[kr][a]) * G.reg_mul[t];
-    uint32_t iff = sel4(s.j0, s.j1, s.j2, s.j3, kf) +
-                   (uint32_t)((int)G.map[kf + 4][b] - (int)G.map[kf + 4][a]) * G.ref_mul[t];
-    int vr = pdb_read_k(P, kr, ir);
-    int vf = pdb_read_k(P, kf, iff);
+    const uint32_t ra = G.wr[t][a], fa = G.wf[t][a];
+    const int kr = (int)(ra >> 28), kf = (int)(fa >> 28);
+    uint32_t ir = sel4(s.i0, s.i1, s.i2, s.i3, kr) + (G.wr[t][b] - ra);
+    uint32_t iff = sel4(s.j0, s.j1, s.j2, s.j3, kf) + (G.wf[t][b] - fa);
+    int vr = pdb_read(P, ir);
+    int vf = pdb_read(P, iff);
     vals = (uint32_t)vr | ((uint32_t)vf << 8);
     int A = s.A - byte_of(s.v0, kr) + vr;
     int B = s.B - byte_of(s.v1, kf) + vf;
@@ -194,9 +196,9 @@ __device__ __forceinline__ int pdb_child_h(const PdbCtx &P, const HState<1> &s, 
 __device__ __forceinline__ void pdb_apply_known(const PdbCtx &P, HState<1> &s, int t, int a, int b,
                                                 uint32_t vals) {
     const PdbGeo &G = *P.geo;
-    int kr = G.reg_k[t], kf = G.ref_k[t] - 4;
-    uint32_t dr = (uint32_t)((int)G.map[kr][b] - (int)G.map[kr][a]) * G.reg_mul[t];
-    uint32_t df = (uint32_t)((int)G.map[kf + 4][b] - (int)G.map[kf + 4][a]) * G.ref_mul[t];
+    const uint32_t ra = G.wr[t][a], fa = G.wf[t][a];
+    const int kr = (int)(ra >> 28), kf = (int)(fa >> 28);
+    const uint32_t dr = G.wr[t][b] - ra, df = G.wf[t][b] - fa;
     add4(s.i0, s.i1, s.i2, s.i3, kr, dr);
     add4(s.j0, s.j1, s.j2, s.j3, kf, df);
     int vr = (int)(vals & 0xffu), vf = (int)((vals >> 8) & 0xffu);
@@ -209,13 +211,13 @@ __device__ __forceinline__ void pdb_apply_known(const PdbCtx &P, HState<1> &s, i
 
 __device__ __forceinline__ void pdb_apply(const PdbCtx &P, HState<1> &s, int t, int a, int b) {
     const PdbGeo &G = *P.geo;
-    int kr = G.reg_k[t], kf = G.ref_k[t] - 4;
-    uint32_t dr = (uint32_t)((int)G.map[kr][b] - (int)G.map[kr][a]) * G.reg_mul[t];
-    uint32_t df = (uint32_t)((int)G.map[kf + 4][b] - (int)G.map[kf + 4][a]) * G.ref_mul[t];
+    const uint32_t ra = G.wr[t][a], fa = G.wf[t][a];
+    const int kr = (int)(ra >> 28), kf = (int)(fa >> 28);
+    const uint32_t dr = G.wr[t][b] - ra, df = G.wf[t][b] - fa;
     add4(s.i0, s.i1, s.i2, s.i3, kr, dr);
     add4(s.j0, s.j1, s.j2, s.j3, kf, df);
-    int vr = pdb_read_k(P, kr, sel4(s.i0, s.i1, s.i2, s.i3, kr));
-    int vf = pdb_read_k(P, kf, sel4(s.j0, s.j1, s.j2, s.j3, kf));
+    int vr = pdb_read(P, sel4(s.i0, s.i1, s.i2, s.i3, kr));
+    int vf = pdb_read(P, sel4(s.j0, s.j1, s.j2, s.j3, kf));
     s.A += vr - byte_of(s.v0, kr);
     s.B += vf - byte_of(s.v1, kf);
     s.v0 = set_byte(s.v0, kr, vr);
@@ -228,21 +230,20 @@ __device__ __forceinline__ void pdb_apply(const PdbCtx &P, HState<1> &s, int t, 
 // regular and the reflected lookup containing its tile.
 __device__ __forceinline__ void pdb_init(const PdbCtx &P, HState<1> &s, const Board<5> &bd) {
     const PdbGeo &G = *P.geo;
-    s.i0 = s.i1 = s.i2 = s.i3 = 0;
-    s.j0 = s.j1 = s.j2 = s.j3 = 0;
+    s.i0 = s.j0 = 0u; // lookups 0 and 4: table 0
+    s.i1 = s.i2 = s.i3 = (uint32_t)kPdbDirect;
+    s.j1 = s.j2 = s.j3 = (uint32_t)kPdbDirect;
 #pragma unroll
     for (int c = 0; c < 25; ++c) {
         int t = bd.tile(c);
         if (t != 0) {
-            int kr = G.reg_k[t], kf = G.ref_k[t] - 4;
-            add4(s.i0, s.i1, s.i2, s.i3, kr, (uint32_t)G.map[kr][c] * G.reg_mul[t]);
-            add4(s.j0, s.j1, s.j2, s.j3, kf, (uint32_t)G.map[kf + 4][c] * G.ref_mul[t]);
+            const uint32_t wr = G.wr[t][c], wf = G.wf[t][c];
+            add4(s.i0, s.i1, s.i2, s.i3, (int)(wr >> 28), wr & kWeightMask);
+            add4(s.j0, s.j1, s.j2, s.j3, (int)(wf >> 28), wf & kWeightMask);
         }
     }
-    int a0 = pdb_read_k(P, 0, s.i0), a1 = pdb_read_k(P, 1, s.i1), a2 = pdb_read_k(P, 2, s.i2),
-        a3 = pdb_read_k(P, 3, s.i3);
-    int b0 = pdb_read_k(P, 0, s.j0), b1 = pdb_read_k(P, 1, s.j1), b2 = pdb_read_k(P, 2, s.j2),
-        b3 = pdb_read_k(P, 3, s.j3);
+    int a0 = pdb_read(P, s.i0), a1 = pdb_read(P, s.i1), a2 = pdb_read(P, s.i2), a3 = pdb_read(P, s.i3);
+    int b0 = pdb_read(P, s.j0), b1 = pdb_read(P, s.j1), b2 = pdb_read(P, s.j2), b3 = pdb_read(P, s.j3);
     s.v0 = (uint32_t)a0 | ((uint32_t)a1 << 8) | ((uint32_t)a2 << 16) | ((uint32_t)a3 << 24);
     s.v1 = (uint32_t)b0 | ((uint32_t)b1 << 8) | ((uint32_t)b2 << 16) | ((uint32_t)b3 << 24);
     s.A = a0 + a1 + a2 + a3;
@@ -287,6 +288,42 @@ __device__ __forceinline__ void do_move_known(Pos<W, 1> &s, int d, const PdbCtx 
     s.bd.slide(t, from, to);
     s.br = nr;
     s.bc = nc;
+}
+
+// Blank move in direction d for the PDB search.  The moved tile's two new
+// lookup values are either known (kv, packed regular | reflected << 8, from
+// the parent's f-test) or read from the tables.  Returns the two values the
+// move replaced, packed the same way, for the stack entry that undoes it.
+template <int W>
+__device__ __forceinline__ uint32_t move_pdb(Pos<W, 1> &s, int d, const PdbCtx &P, bool known, uint32_t kv) {
+    const int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
+    const int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
+    const int from = nr * W + nc, to = s.br * W + s.bc;
+    const int t = s.bd.tile(from);
+    const PdbGeo &G = *P.geo;
+    const uint32_t ra = G.wr[t][from], fa = G.wf[t][from];
+    const int kr = (int)(ra >> 28), kf = (int)(fa >> 28);
+    HState<1> &h = s.hs;
+    add4(h.i0, h.i1, h.i2, h.i3, kr, G.wr[t][to] - ra);
+    add4(h.j0, h.j1, h.j2, h.j3, kf, G.wf[t][to] - fa);
+    const int oldr = byte_of(h.v0, kr), oldf = byte_of(h.v1, kf);
+    int vr, vf;
+    if (known) {
+        vr = (int)(kv & 0xffu);
+        vf = (int)((kv >> 8) & 0xffu);
+    } else {
+        vr = pdb_read(P, sel4(h.i0, h.i1, h.i2, h.i3, kr));
+        vf = pdb_read(P, sel4(h.j0, h.j1, h.j2, h.j3, kf));
+    }
+    h.A += vr - oldr;
+    h.B += vf - oldf;
+    h.v0 = set_byte(h.v0, kr, vr);
+    h.v1 = set_byte(h.v1, kf, vf);
+    h.h = h.A > h.B ? h.A : h.B;
+    s.bd.slide(t, from, to);
+    s.br = nr;
+    s.bc = nc;
+    return (uint32_t)oldr | ((uint32_t)oldf << 8);
 }
 
 // values, before the move, of the two lookups containing the tile that a

@@ -12,7 +12,9 @@ namespace spz {
 enum : int { DIR_UP = 0, DIR_RIGHT = 1, DIR_LEFT = 2, DIR_DOWN = 3, DIR_NONE = 4 };
 
 // Lane states of the DFS engine.
-enum : int { ST_IDLE = 0, ST_FRESH = 1, ST_POP = 2, ST_DRAINED = 3 };
+// lane states; ST_DESC (children f-tested, descent into the first passing one
+// pending) never outlives a round
+enum : int { ST_IDLE = 0, ST_FRESH = 1, ST_POP = 2, ST_DRAINED = 3, ST_DESC = 4 };
 
 constexpr int kMaxIters = 128;      // threshold iterations kept per instance
 constexpr int kFBins = 256;         // f-value histogram bins of the top tree
@@ -106,9 +108,9 @@ struct KParams {
     uint32_t *c_pdb;   // [10][n_lanes]: 8 lookup indices + 2 packed value words
     uint8_t *c_stack;  // [maxd][n_lanes]
 
-    // pattern databases: two canonical tables, direct 25^6 addressing
-    const uint8_t *pdb0;
-    const uint8_t *pdb1;
+    // pattern databases: the two canonical tables in direct 25^6 addressing,
+    // laid end to end (table 1 at kPdbDirect)
+    const uint8_t *pdb;
 };
 
 } // namespace spz

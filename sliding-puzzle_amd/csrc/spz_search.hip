@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
     ST *cstk = (ST *)p.c_stack;
     const int sstride = nw * 64;
 
-    PdbCtx P{&geo_s, p.pdb0, p.pdb1};
+    PdbCtx P{&geo_s, p.pdb};
     if constexpr (HEUR == 1) {
         const uint32_t *src = (const uint32_t *)geo_g;
         uint32_t *dst = (uint32_t *)&geo_s;
@@ -186,6 +186,22 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
         if (mode == 1 && p.found[inst] != kNoGoal) st = ST_POP; // first-solution: abandon
     };
 
+    // move down into child d (stack level sl, R = passing children still to
+    // visit there); for the PDB the child's values are known or read
+    uint32_t dpass = 0, dval = 0, dval2 = 0; // pending descent (ST_DESC)
+    auto enter_child = [&](int d, uint32_t Rr, int sl, bool kn, uint32_t kvv) {
+        uint32_t ov = 0;
+        if constexpr (HEUR == 1)
+            ov = move_pdb<W>(s, d, P, kn, kvv);
+        else
+            do_move<W, HEUR>(s, d, P);
+        stk[sl * sstride] = SE::make(d, Rr, ov);
+        sp = sl + 1;
+        ++g;
+        prev = d;
+        st = ST_FRESH;
+    };
+
     uint32_t pf_idle = 0, pf_exp = 0, pf_pop = 0, pf_steps = 0;
     for (int step = 0; step < budget; ++step) {
         // ---- idle lanes take queue items (one atomic per wave) ----
@@ -206,11 +222,16 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                 }
             }
         }
-        if (!__ballot(st == ST_FRESH || st == ST_POP)) break;
+        if (!__ballot(st == ST_FRESH || st == ST_POP || st == ST_DESC)) break;
 
-        // ---- pop: undo up to kPops moves, then enter the next sibling ----
+        // ---- pop: undo up to kPops moves, until a level with a passing
+        // child not visited yet ----
         bool finish = false;
-        int sib = -1;
+        int down = -1;      // child entered this step (-1: none)
+        int slot = 0;       // its stack level
+        uint32_t R = 0;     // passing children still to visit at that level
+        bool known = false; // its lookup values are known (PDB)
+        uint32_t kv = 0;
 #pragma unroll 1
         for (int pk = 0; pk < kPops && st == ST_POP; ++pk) {
             if (sp == 0) {
@@ -219,29 +240,31 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
             }
             uint32_t e = stk[(sp - 1) * sstride];
             int d = SE::dir(e);
-            uint32_t R = SE::rest(e);
+            uint32_t Re = SE::rest(e);
             if constexpr (HEUR == 1)
                 do_move_known<W>(s, 3 - d, P, SE::vals(e));
             else
                 do_move<W, HEUR>(s, 3 - d, P);
             --g;
-            if (R) {
-                int d2 = __builtin_ctz(R);
-                R &= R - 1;
-                uint32_t ov = 0;
-                if constexpr (HEUR == 1) ov = pdb_moved_vals<W>(s, d2, P);
-                stk[(sp - 1) * sstride] = SE::make(d2, R, ov);
-                sib = d2;
+            if (Re) {
+                down = __builtin_ctz(Re);
+                R = Re & (Re - 1);
+                slot = sp - 1;
                 st = ST_FRESH;
             } else {
                 --sp;
             }
         }
-        if (sib >= 0) {
-            do_move<W, HEUR>(s, sib, P);
-            ++g;
-            prev = sib;
+        // ---- or enter the first passing child of last step's expansion ----
+        if (st == ST_DESC) {
+            down = __builtin_ctz(dpass);
+            R = dpass & (dpass - 1);
+            slot = sp;
+            known = true;
+            kv = (((down >= 2) ? dval2 : dval) >> (16 * (down & 1))) & 0xffffu;
         }
+        // ---- one move down per lane and step, both cases on one path ----
+        if (down >= 0) enter_child(down, R, slot, known, kv);
         if (finish) {
             if (cnt) {
                 atomicAdd(&p.cnt[inst], (unsigned long long)cnt);
@@ -261,12 +284,9 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
             pf_pop += (st == ST_POP) ? 1u : 0u;
         }
         // ---- expand: f-tests of the children ----
-        bool xp = false;
-        uint32_t pass = 0;
-        uint32_t cvals = 0, cvals2 = 0; // PDB values of children 0,1 / 2,3 (16 bits each)
         if (st == ST_FRESH) {
             const int h = s.hs.h;
-            xp = true;
+            bool xp = true;
             if (h == 0) {
                 atomicMin(&p.found[inst], g);
                 if (mode == 1) {
@@ -280,7 +300,8 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                                  (br < W - 1 ? 8u : 0u);
                 uint32_t gen = legal & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
                 cnt += (uint32_t)__builtin_popcount(gen);
-                pass = gen;
+                uint32_t pass = gen;
+                uint32_t cvals = 0, cvals2 = 0; // PDB values of children 0,1 / 2,3 (16 bits each)
                 const int bl = br * W + bc;
                 if constexpr (HEUR == 0) {
                     // Manhattan is consistent: h changes by exactly +-1 per
@@ -320,27 +341,14 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                         }
                     }
                 }
-            }
-        }
-
-        // ---- expand: descend into the first passing child ----
-        if (xp) {
-            if (pass) {
-                if (sp < MAXD) {
-                    int d = __builtin_ctz(pass);
-                    uint32_t R = pass & (pass - 1);
-                    uint32_t ov = 0;
-                    if constexpr (HEUR == 1) ov = pdb_moved_vals<W>(s, d, P);
-                    stk[sp * sstride] = SE::make(d, R, ov);
-                    ++sp;
-                    if constexpr (HEUR == 1) {
-                        uint32_t w2 = (d >= 2) ? cvals2 : cvals;
-                        do_move_known<W>(s, d, P, (w2 >> (16 * (d & 1))) & 0xffffu);
-                    } else {
-                        do_move<W, HEUR>(s, d, P);
-                    }
-                    ++g;
-                    prev = d;
+                if (!pass) {
+                    st = ST_POP;
+                } else if (sp < MAXD) {
+                    // enter the first passing child next step
+                    st = ST_DESC;
+                    dpass = pass;
+                    dval = cvals;
+                    dval2 = cvals2;
                 } else {
                     // stack full: the passing children become queue items
                     int k = __builtin_popcount(pass);
@@ -356,12 +364,15 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                     }
                     st = ST_POP;
                 }
-            } else {
-                st = ST_POP;
             }
         }
     }
 
+    // a descent still pending is made now, so the saved context is a node
+    if (st == ST_DESC) {
+        const int dn = __builtin_ctz(dpass);
+        enter_child(dn, dpass & (dpass - 1), sp, true, (((dn >= 2) ? dval2 : dval) >> (16 * (dn & 1))) & 0xffffu);
+    }
 
     // ---- round end: donate shallow open nodes, then save the context ----
     const bool busy = (st == ST_FRESH || st == ST_POP);
@@ -640,7 +651,7 @@ __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *ge
                                                      unsigned *nfront, unsigned *topgen, int *top_goal,
                                                      int *frozen, int depth, int max_depth, unsigned target) {
     __shared__ PdbGeo geo_s;
-    PdbCtx P{&geo_s, p.pdb0, p.pdb1};
+    PdbCtx P{&geo_s, p.pdb};
     if constexpr (HEUR == 1) {
         const uint32_t *src = (const uint32_t *)geo_g;
         uint32_t *dst = (uint32_t *)&geo_s;
@@ -701,7 +712,7 @@ __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *ge
 template <int W, int HEUR>
 __global__ void k_eval_h(KParams p, const PdbGeo *geo_g, Item *items, int n) {
     __shared__ PdbGeo geo_s;
-    PdbCtx P{&geo_s, p.pdb0, p.pdb1};
+    PdbCtx P{&geo_s, p.pdb};
     if constexpr (HEUR == 1) {
         const uint32_t *src = (const uint32_t *)geo_g;
         uint32_t *dst = (uint32_t *)&geo_s;
