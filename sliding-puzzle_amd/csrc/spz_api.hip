@@ -163,13 +163,11 @@ void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic) {
     cfg->queue_items = 0;
 }
 
-// Steps per lane and round when the configuration leaves it open.  The PDB
-// search (random table gathers) runs fastest with shorter rounds, which
-// refill idle lanes sooner; the Manhattan search (ALU only) with longer ones,
-// which spend less on the context save and restore (measured on MI355X:
-// 24-puzzle walks of 160 moves, 512 vs 1024 steps: +6%; 15-puzzle batch of
-// 4096: 1024 vs 512: +1%).
-static int default_round_budget(int heur) { return heur == SPZ_H_PDB666 ? 512 : 1024; }
+// Steps per lane and round when the configuration leaves it open: shorter
+// rounds refill idle lanes sooner, longer ones spend less on the context save
+// and restore (measured on MI355X, both heuristics: 1024 beats 256 and 512,
+// 2048 is level with it).
+static int default_round_budget(int) { return 1024; }
 
 // Lane count (from occupancy unless given) and lane-context arrays for the
 // context's workgroup size and LDS stack depth.
