@@ -15,7 +15,7 @@
 
 namespace spz {
 hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo, int wg, hipStream_t s);
-size_t round_lds_bytes(int maxd, int wg, int heur);
+size_t round_lds_bytes(int width, int maxd, int wg, int heur);
 const void *round_kernel_ptr(int width, int heur);
 hipError_t launch_restart(const KParams &p, hipStream_t s);
 hipError_t launch_control(const KParams &p, hipStream_t s);
@@ -169,8 +169,19 @@ void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic) {
 // 2048 is level with it).
 static int default_round_budget(int) { return 1024; }
 
-// Lane count (from occupancy unless given) and lane-context arrays for the
-// context's workgroup size and LDS stack depth.
+// Lanes per compute unit of the 24-puzzle searches: more resident lanes than
+// this lose more to L2 sharing and finer work splitting than they gain in
+// latency hiding (measured on MI355X: 768 beat 1024 and 512 for both
+// heuristics).
+constexpr int kLanesPerCu5 = 768;
+// Lanes per initial work item (root-split frontier node) when the lane count
+// is automatic: a batch with little work runs on fewer lanes, which split it
+// less finely (a single 15-puzzle instance, 2048 frontier nodes: 65536 lanes
+// ran 2.4x as fast as 524288, and faster than 32768 or 131072).
+constexpr size_t kLanesPerItem = 32;
+
+// Lane capacity (from occupancy unless given) and lane-context arrays for
+// the context's workgroup size and LDS stack depth.
 static int configure_lanes(spz_ctx *ctx) {
     const spz_config *cfg = &ctx->cfg;
     const int wg = ctx->wg;
@@ -179,8 +190,9 @@ static int configure_lanes(spz_ctx *ctx) {
     if (lanes <= 0) {
         int blocks = 0;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, round_kernel_ptr(cfg->width, cfg->heuristic), wg,
-                                                         round_lds_bytes(cfg->lds_stack_depth, wg, cfg->heuristic));
+                                                         round_lds_bytes(cfg->width, cfg->lds_stack_depth, wg, cfg->heuristic));
         if (e != hipSuccess || blocks <= 0) blocks = 2;
+        if (cfg->width == 5) blocks = std::max(1, std::min(blocks, kLanesPerCu5 / wg));
         lanes = ctx->n_cu * blocks * wg;
     }
     lanes = (lanes + wg - 1) / wg * wg;
@@ -583,8 +595,18 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     const int capl = (int)std::max<unsigned>(4u, 4u * target);
     const size_t N = (size_t)n;
     const size_t front_items = N * (size_t)capl;
+    // lanes of this solve: all of the context's, or (automatic count) as many
+    // as the batch's initial work keeps useful, at least one workgroup per CU
+    size_t lanes = (size_t)ctx->n_lanes;
+    if (ctx->cfg.lanes <= 0) {
+        const size_t wg = (size_t)ctx->wg;
+        size_t want = kLanesPerItem * N * (size_t)std::max(1u, target);
+        want = std::max(want, (size_t)ctx->n_cu * wg);
+        want = (want + wg - 1) / wg * wg;
+        lanes = std::min(lanes, want);
+    }
     size_t qcap = ctx->cfg.queue_items > 0 ? (size_t)ctx->cfg.queue_items
-                                           : std::max<size_t>(1u << 22, 4 * front_items + 8 * (size_t)ctx->n_lanes);
+                                           : std::max<size_t>(1u << 22, 4 * front_items + 8 * lanes);
     qcap = next_pow2(qcap);
     ctx->qcap = qcap;
 
@@ -618,6 +640,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     }
 
     KParams p = base_params(ctx, n, max_iters, capl);
+    p.n_lanes = (int)lanes;
 
     // roots and their heuristic
     std::vector<Item> roots(N);
@@ -691,7 +714,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         ctl[5] = ctl[10];
         HIPCHK(hipMemcpyAsync(ctx->ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemsetAsync(ctx->q_ctl.p, 0, 64, s));
-        HIPCHK(hipMemsetAsync(ctx->c_st.p, 0, (size_t)ctx->n_lanes * 4, s));
+        HIPCHK(hipMemsetAsync(ctx->c_st.p, 0, lanes * 4, s));
         HIPCHK(hipStreamSynchronize(s));
     }
 
@@ -797,7 +820,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     ctx->stats[9] = (double)timed;
     ctx->stats[0] = (double)h_ctl[6];
     ctx->stats[2] = (double)h_ctl[7];
-    ctx->stats[3] = (double)ctx->n_lanes;
+    ctx->stats[3] = (double)lanes;
     ctx->stats[7] = (double)h_ctl[8];
     hipHostFree(h_ctl);
 

@@ -27,7 +27,7 @@ template <> struct Geo<5> {
     static constexpr int MAXD = 160;
     static constexpr int DEFD = 96;
 };
-// default LDS stack depth of the PDB search (its entries are two bytes)
+// default LDS stack depth of the PDB search (two-byte entries)
 constexpr int kDefDepthPdb = 64;
 
 // goal row / column of tile t (tile t belongs at cell t)
@@ -92,6 +92,41 @@ template <> struct Board<5> {
 };
 
 // host helper: board from a row-major byte array
+// 5x5 board of the search kernel, held in LDS: one byte per cell, the
+// cells of a workgroup's lanes interleaved (cell c of lane x at
+// base[c * stride + x]), so reading the tile next to the blank is one LDS
+// load and a slide two byte stores.  Copying it copies the reference, not
+// the cells: regs() gives a register copy.
+struct LdsBoard5 {
+    uint8_t *p;  // this lane's cell 0
+    int stride;  // lanes per workgroup
+    __device__ __forceinline__ int tile(int c) const { return (int)p[c * stride]; }
+    __device__ __forceinline__ void slide(int t, int from, int to) {
+        p[to * stride] = (uint8_t)t;
+        p[from * stride] = 0;
+    }
+    __device__ __forceinline__ void unpack(const uint32_t (&o)[4]) {
+        Board<5> b;
+        b.unpack(o);
+#pragma unroll
+        for (int c = 0; c < 25; ++c) p[c * stride] = (uint8_t)b.tile(c);
+    }
+    __device__ __forceinline__ Board<5> regs() const {
+        Board<5> b;
+        b.lo = b.hi = 0ull;
+        b.top = 0u;
+#pragma unroll
+        for (int c = 0; c < 25; ++c) b.add(c, tile(c));
+        return b;
+    }
+    __device__ __forceinline__ void pack(uint32_t (&o)[4]) const { regs().pack(o); }
+};
+
+// board type of the search kernel's lanes
+template <int W> struct HotBoard;
+template <> struct HotBoard<4> { using T = Board<4>; };
+template <> struct HotBoard<5> { using T = LdsBoard5; };
+
 template <int W> inline void board_from_bytes(const uint8_t *b, Board<W> &o);
 template <> inline void board_from_bytes<4>(const uint8_t *b, Board<4> &o) {
     o.b = 0;
@@ -228,7 +263,7 @@ __device__ __forceinline__ void pdb_apply(const PdbCtx &P, HState<1> &s, int t, 
 // Full (non-incremental) initialisation of the PDB state from a board:
 // every non-blank cell adds map_k(cell) * 25^(5-slot) to the index of the
 // regular and the reflected lookup containing its tile.
-__device__ __forceinline__ void pdb_init(const PdbCtx &P, HState<1> &s, const Board<5> &bd) {
+template <class BD> __device__ __forceinline__ void pdb_init(const PdbCtx &P, HState<1> &s, const BD &bd) {
     const PdbGeo &G = *P.geo;
     s.i0 = s.j0 = 0u; // lookups 0 and 4: table 0
     s.i1 = s.i2 = s.i3 = (uint32_t)kPdbDirect;
@@ -252,16 +287,16 @@ __device__ __forceinline__ void pdb_init(const PdbCtx &P, HState<1> &s, const Bo
 }
 
 // Lane position: board + blank (cell, row, col) + heuristic state.
-template <int W, int HEUR> struct Pos {
-    Board<W> bd;
+template <int W, int HEUR, class BD = Board<W>> struct Pos {
+    BD bd;
     int br, bc;
     HState<HEUR> hs;
 };
 
 // Move the blank one cell in direction d: the tile at the neighbour slides
 // into the blank.
-template <int W, int HEUR>
-__device__ __forceinline__ void do_move(Pos<W, HEUR> &s, int d, const PdbCtx &P) {
+template <int W, int HEUR, class BD>
+__device__ __forceinline__ void do_move(Pos<W, HEUR, BD> &s, int d, const PdbCtx &P) {
     int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
     int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
     int from = nr * W + nc, to = s.br * W + s.bc;
@@ -278,8 +313,8 @@ __device__ __forceinline__ void do_move(Pos<W, HEUR> &s, int d, const PdbCtx &P)
 }
 
 // do_move for the PDB state with the moved tile's two lookup values known
-template <int W>
-__device__ __forceinline__ void do_move_known(Pos<W, 1> &s, int d, const PdbCtx &P, uint32_t vals) {
+template <int W, class BD>
+__device__ __forceinline__ void do_move_known(Pos<W, 1, BD> &s, int d, const PdbCtx &P, uint32_t vals) {
     int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
     int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
     int from = nr * W + nc, to = s.br * W + s.bc;
@@ -294,8 +329,8 @@ __device__ __forceinline__ void do_move_known(Pos<W, 1> &s, int d, const PdbCtx 
 // lookup values are either known (kv, packed regular | reflected << 8, from
 // the parent's f-test) or read from the tables.  Returns the two values the
 // move replaced, packed the same way, for the stack entry that undoes it.
-template <int W>
-__device__ __forceinline__ uint32_t move_pdb(Pos<W, 1> &s, int d, const PdbCtx &P, bool known, uint32_t kv) {
+template <int W, class BD>
+__device__ __forceinline__ uint32_t move_pdb(Pos<W, 1, BD> &s, int d, const PdbCtx &P, bool known, uint32_t kv) {
     const int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
     const int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
     const int from = nr * W + nc, to = s.br * W + s.bc;
@@ -328,8 +363,8 @@ __device__ __forceinline__ uint32_t move_pdb(Pos<W, 1> &s, int d, const PdbCtx &
 
 // values, before the move, of the two lookups containing the tile that a
 // blank move in direction d slides (packed regular | reflected << 8)
-template <int W>
-__device__ __forceinline__ uint32_t pdb_moved_vals(const Pos<W, 1> &s, int d, const PdbCtx &P) {
+template <int W, class BD>
+__device__ __forceinline__ uint32_t pdb_moved_vals(const Pos<W, 1, BD> &s, int d, const PdbCtx &P) {
     int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
     int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
     int t = s.bd.tile(nr * W + nc);
@@ -369,8 +404,8 @@ template <> struct StackEntry<1> {
     static __device__ __forceinline__ uint32_t vals(uint32_t e) { return ((e >> 4) & 63u) | (((e >> 10) & 63u) << 8); }
 };
 
-template <int W, int HEUR>
-__device__ __forceinline__ void init_heur(Pos<W, HEUR> &s, const PdbCtx &P, uint32_t meta) {
+template <int W, int HEUR, class BD>
+__device__ __forceinline__ void init_heur(Pos<W, HEUR, BD> &s, const PdbCtx &P, uint32_t meta) {
     if constexpr (HEUR == 0) {
         s.hs.h = meta_h(meta);
     } else {

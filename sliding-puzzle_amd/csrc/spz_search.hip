@@ -65,8 +65,8 @@ __device__ void finish_iteration(const KParams &p, int inst) {
 }
 
 // Append one item.  pos is a reserved queue position.
-template <int W, int HEUR>
-__device__ __forceinline__ void put_item(const KParams &p, unsigned long long pos, const Pos<W, HEUR> &s, int g,
+template <int W, int HEUR, class BD>
+__device__ __forceinline__ void put_item(const KParams &p, unsigned long long pos, const Pos<W, HEUR, BD> &s, int g,
                                          int prev, int inst, uint32_t origin) {
     Item it;
     s.bd.pack(it.b);
@@ -77,14 +77,28 @@ __device__ __forceinline__ void put_item(const KParams &p, unsigned long long po
     p.q[pos & p.qmask] = it;
 }
 
-template <int W, int HEUR>
-__device__ __forceinline__ void pos_from_item(Pos<W, HEUR> &s, const uint32_t (&b)[4], uint32_t meta,
+template <int W, int HEUR, class BD>
+__device__ __forceinline__ void pos_from_item(Pos<W, HEUR, BD> &s, const uint32_t (&b)[4], uint32_t meta,
                                               const PdbCtx &P) {
     s.bd.unpack(b);
     int blank = meta_blank(meta);
     s.br = blank / W;
     s.bc = blank - s.br * W;
     init_heur<W, HEUR>(s, P, meta);
+}
+
+// register copy of a lane position (the search kernel's 5x5 board is in LDS)
+template <int W, int HEUR, class BD>
+__device__ __forceinline__ Pos<W, HEUR> reg_pos(const Pos<W, HEUR, BD> &s) {
+    Pos<W, HEUR> c;
+    if constexpr (W == 5)
+        c.bd = s.bd.regs();
+    else
+        c.bd = s.bd;
+    c.br = s.br;
+    c.bc = s.bc;
+    c.hs = s.hs;
+    return c;
 }
 
 // ---------------------------------------------------------------------------
@@ -125,8 +139,14 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
     uint32_t stw = p.c_st[gl];
     int st = (int)(stw & 0xff);
     int sp = (int)(stw >> 8);
-    Pos<W, HEUR> s;
-    s.bd.unpack({0u, 0u, 0u, 0u});
+    // stack levels, then (width 5) the lanes' boards
+    Pos<W, HEUR, typename HotBoard<W>::T> s;
+    if constexpr (W == 5) {
+        s.bd.p = lds + (size_t)MAXD * sstride * sizeof(ST) + threadIdx.x;
+        s.bd.stride = blockDim.x;
+    } else {
+        s.bd.unpack({0u, 0u, 0u, 0u});
+    }
     s.br = s.bc = 0;
     s.hs.h = 0;
     uint32_t root[4] = {0, 0, 0, 0};
@@ -370,7 +390,7 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                     while (pass) {
                         int d = __builtin_ctz(pass);
                         pass &= pass - 1;
-                        Pos<W, HEUR> c = s;
+                        Pos<W, HEUR> c = reg_pos(s);
                         do_move<W, HEUR>(c, d, P);
                         put_item<W, HEUR>(p, pos++, c, g + 1, d, inst, origin);
                     }
@@ -759,9 +779,11 @@ __global__ void k_scatter_roots(const Item *roots, Item *lvl, int capl, int n) {
 
 // ---------------------------------------------------------------------------
 // host-side launchers
+size_t round_lds_bytes(int width, int maxd, int wg, int heur);
+
 template <int W, int HEUR>
 static hipError_t launch_round_t(const KParams &p, const PdbGeo *geo, int wg, hipStream_t s) {
-    size_t lds = (size_t)p.maxd * wg * sizeof(typename StackEntry<HEUR>::T);
+    size_t lds = round_lds_bytes(W, p.maxd, wg, HEUR);
     hipLaunchKernelGGL((k_round<W, HEUR>), dim3(p.n_lanes / wg), dim3(wg), lds, s, p, geo);
     return hipGetLastError();
 }
@@ -772,7 +794,10 @@ hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo
     return launch_round_t<5, 1>(p, geo, wg, s);
 }
 
-size_t round_lds_bytes(int maxd, int wg, int heur) { return (size_t)maxd * wg * (heur == 1 ? 2 : 1); }
+// dynamic LDS of k_round: the stack levels, then (width 5) the 25-cell boards
+size_t round_lds_bytes(int width, int maxd, int wg, int heur) {
+    return (size_t)maxd * wg * (heur == 1 ? 2 : 1) + (width == 5 ? (size_t)25 * wg : 0);
+}
 
 const void *round_kernel_ptr(int width, int heur) {
     if (width == 4) return (const void *)&k_round<4, 0>;
