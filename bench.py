@@ -152,7 +152,7 @@ def cpu_baseline(width, heur, boards, res, solver, budget_s=20.0):
 
     order = sorted(range(len(boards)), key=lambda i: res[i].total_nodes)
     mean_nodes = sum(r.total_nodes for r in res) / len(res)
-    est_rate = 8e6 if heur == spz.H_PDB666 else 60e6
+    est_rate = 30e6 if heur == spz.H_PDB666 else 60e6  # nodes/s of the CPU solvers, one core
     pick, acc = [], 0
     for i in order:
         if pick and acc + res[i].total_nodes > est_rate * budget_s:

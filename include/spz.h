@@ -78,7 +78,8 @@ typedef struct spz_config {
     int32_t lb_policy;       /* SPZ_LB_* */
     int32_t device;          /* HIP device ordinal */
     int32_t time_kernels;    /* 1: time every search-round launch with HIP
-                                events on the solver's stream (stats 8-9) */
+                                events on the solver's stream (stats 8-9);
+                                the rounds then run without the hipGraph */
     int32_t use_graph;       /* 1: replay the search rounds from a captured
                                 hipGraph (default); 0: launch them one by one */
     int32_t log_samples;     /* >0: keep up to this many (frontier node,
@@ -167,6 +168,9 @@ int spz_mlp_set_params(spz_ctx *ctx, const float *params);
 int spz_mlp_get_params(spz_ctx *ctx, float *params);
 /* forward pass on the GPU: out[i] for boards[i] at threshold thr[i] */
 int spz_mlp_forward(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, int32_t n, float *out);
+/* the frontier predictor of SPZ_LB_LEARNED (one fused MFMA kernel): out[i*16 + m]
+ * for boards[i] at threshold thr0[i] + 2m, m = 0..15 (out holds 16 n floats) */
+int spz_mlp_predict(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr0, int32_t n, float *out);
 /* loss = mean (y - target)^2 and its gradient w.r.t. the parameters */
 int spz_mlp_grad(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const float *target, int32_t n,
                  float *loss, float *grad);

@@ -684,7 +684,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     HIPCHK(hipMemsetAsync(ctx->nsamp.p, 0, 64, s));
     if (ctx->cfg.lb_policy == SPZ_LB_LEARNED)
         HIPCHK(mlp_predict_frontier(ctx->mlp, w, ctx->frontier.as<Item>(), ctx->nfront.as<unsigned>(),
-                                    ctx->roots.as<Item>(), n, capl, kPredK, ctx->pred.as<float>(), s));
+                                    ctx->roots.as<Item>(), n, capl, ctx->pred.as<float>(), s));
     HIPCHK(hipStreamSynchronize(s));
     ctx->stats[10] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_split).count();
 
@@ -759,7 +759,9 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     // all their state from device memory, so replays need no new arguments
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    bool use_graph = ctx->cfg.use_graph != 0;
+    // kernel timing runs the rounds eagerly: HIP events recorded by a captured
+    // graph cannot be timed against each other
+    bool use_graph = ctx->cfg.use_graph != 0 && !timing;
     if (use_graph) {
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
         hipError_t e2 = e == hipSuccess ? enqueue_period() : e;
@@ -952,6 +954,23 @@ int spz_mlp_forward(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, int
     if (rc) return rc;
     HIPCHK(mlp_forward(ctx->mlp, n, ctx->stream));
     HIPCHK(hipMemcpyAsync(out, ctx->mlp.y, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return SPZ_OK;
+}
+
+static_assert(kPredK == kPredThr, "one prediction per threshold slot of the restart");
+
+int spz_mlp_predict(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr0, int32_t n, float *out) {
+    if (!ctx || !boards || !thr0 || !out || n <= 0) return SPZ_E_ARG;
+    if (!ctx->mlp_alloced || !ctx->mlp.ready) return fail(ctx, SPZ_E_ARG, "no MLP parameters");
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    DBuf items, thr_d, y;
+    int rc = mlp_load_rows(ctx, boards, thr0, n, items, thr_d);
+    if (rc) return rc;
+    HIPCHK(y.ensure((size_t)n * kPredThr * 4));
+    HIPCHK(mlp_predict_items(ctx->mlp, ctx->cfg.width, items.as<Item>(), thr_d.as<int>(), n, y.as<float>(),
+                             ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, y.p, (size_t)n * kPredThr * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return SPZ_OK;
 }

@@ -47,6 +47,7 @@ struct MlpState {
     float *dz = nullptr;        // [cap][256] scratch gradient
     uint16_t *dzb = nullptr;    // bf16 copy
     float *loss = nullptr;      // [1]
+    uint16_t *w2frag = nullptr; // W2 in MFMA B-fragment order (prediction)
 };
 
 hipError_t mlp_alloc(MlpState &m);
@@ -64,9 +65,16 @@ hipError_t mlp_forward(MlpState &m, int n, hipStream_t s);
 hipError_t mlp_backward(MlpState &m, const float *target, int n, hipStream_t s);
 hipError_t mlp_adam_step(MlpState &m, float lr, hipStream_t s);
 
-// predictions for every frontier node of every instance at K thresholds:
-// pred[(inst*capl + i)*K + k] = y(frontier node i, L = h0[inst] + 2k)
+// Predictions at kPredThr consecutive thresholds L0, L0+2, ... per node, by
+// one fused kernel (one wave per node; the thresholds of a node are the 16
+// rows of one MFMA tile and share layer 1 up to a rank-1 term).
+constexpr int kPredThr = 16;
+// every frontier node of every instance:
+// pred[(inst*capl + i)*kPredThr + k] = y(frontier node i, L = h0[inst] + 2k)
 hipError_t mlp_predict_frontier(MlpState &m, int width, const Item *frontier, const unsigned *nfront,
-                                const Item *roots, int n_inst, int capl, int K, float *pred, hipStream_t s);
+                                const Item *roots, int n_inst, int capl, float *pred, hipStream_t s);
+// n nodes with h in meta: out[i*kPredThr + k] = y(node i, L = thr0[i] + 2k)
+hipError_t mlp_predict_items(MlpState &m, int width, const Item *items, const int *thr0, int n, float *out,
+                             hipStream_t s);
 
 } // namespace spz

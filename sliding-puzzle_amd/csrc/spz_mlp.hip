@@ -226,6 +226,7 @@ void mlp_free(MlpState &m) {
     fr(m.adam_m);
     fr(m.adam_v);
     fr(m.loss);
+    fr(m.w2frag);
     m = MlpState();
 }
 
@@ -347,69 +348,136 @@ hipError_t mlp_adam_step(MlpState &m, float lr, hipStream_t s) {
     return hipGetLastError();
 }
 
-// rows of one instance's frontier x K thresholds -> features
+// W2 in the B-fragment order of v_mfma_f32_16x16x16_bf16: fragment
+// (kb, nb) of lane l holds W2[kb*16 + 4(l/16) + j][nb*16 + l%16], j = 0..3,
+// so each lane loads its 4 values as one 8-byte word.
+__global__ void k_w2frag(const uint16_t *wbf, uint16_t *frag) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x; // (kb*16 + nb)*64 + lane
+    if (t >= 16 * 16 * 64) return;
+    const int lane = t & 63, f = t >> 6, kb = f >> 4, nb = f & 15;
+    const int q = lane >> 4, r = lane & 15;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        frag[(size_t)t * 4 + j] = wbf[kMlpOffW2 + (size_t)(kb * 16 + q * 4 + j) * kMlpH + nb * 16 + r];
+}
+
+// Fused prediction: one wave per node, its kPredThr = 16 thresholds
+// L0 + 2m (m = 0..15) as the 16 rows of one MFMA tile.  Layer 1 of a row is
+// the node's one-hot and h/32 columns of W1 (shared by the 16 rows) plus
+// (L - h)/32 times row 626 of W1; layers 2 and 3 follow in registers.
+// Waves map to node slots (g, i), g < n_groups, i < stride; a slot is live
+// if i < count[g] (or count is null).  L0 = h of roots[g] if given, else
+// thr0[g*stride + i].  Rounding follows the forward pass: bf16 inputs and
+// weights, fp32 sums, bf16 activations after ReLU.
 template <int W>
-__global__ void k_features_frontier(const Item *F, unsigned nf, int K, int L0, long r0, long nrows, uint16_t *x) {
-    const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= nrows) return;
-    const long r = r0 + row;
-    const unsigned i = (unsigned)(r / K);
-    const int k = (int)(r % K);
-    if (i >= nf) return;
-    const Item it = F[i];
+__global__ void __launch_bounds__(256) k_predict16(const Item *items, int stride, const unsigned *count, int n_groups,
+                                                   const Item *roots, const int *thr0, const uint16_t *wbf,
+                                                   const float *master, const uint16_t *w2frag, float *out) {
+    __shared__ float base_s[4][kMlpH];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long slot = (long)blockIdx.x * 4 + wave;
+    const int g = (int)(slot / stride), i = (int)(slot - (long)g * stride);
+    bool live = g < n_groups;
+    if (live && count) live = (unsigned)i < (count[g] < (unsigned)stride ? count[g] : (unsigned)stride);
+    if (!live) return; // wave-uniform
+    const Item it = items[slot];
     Board<W> bd;
     bd.unpack(it.b);
-    const int h = meta_h(it.meta), L = L0 + 2 * k;
+    const int h = meta_h(it.meta);
+    const int L0 = roots ? meta_h(roots[g].meta) : thr0[slot];
     constexpr int N2 = W * W;
-    for (int c = lane; c < kMlpIn; c += 64) {
-        float v = 0.f;
-        if (c < N2 * N2) {
-            int t = c / N2, cc = c - t * N2;
-            v = (bd.tile(cc) == t) ? 1.f : 0.f;
-        } else if (c == 625) {
-            v = h * (1.f / 32.f);
-        } else if (c == 626) {
-            v = (L - h) * (1.f / 32.f);
+    const float xh = bf2f(f2bf(h * (1.f / 32.f)));
+    // layer 1, shared part: sum of the one-hot rows, the h row and b1
+    float *base = base_s[wave];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+        const int col = cc * 64 + lane;
+        float acc = 0.f;
+#pragma unroll
+        for (int c = 0; c < N2; ++c) acc += bf2f(wbf[kMlpOffW1 + (size_t)(bd.tile(c) * N2 + c) * kMlpH + col]);
+        acc += xh * bf2f(wbf[kMlpOffW1 + (size_t)625 * kMlpH + col]);
+        base[col] = acc + master[kMlpOffB1 + col];
+    }
+    // base is this wave's own: LDS accesses of a wave complete in order, so
+    // only the compiler must keep the reads after the writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // layer 1 rows in A-fragment order: lane l holds row m = l%16 (threshold
+    // L0 + 2m), columns kb*16 + 4(l/16) + j
+    const int r = lane & 15, q = lane >> 4;
+    const float xs = bf2f(f2bf((L0 + 2 * r - h) * (1.f / 32.f)));
+    bf16x4 a[16];
+#pragma unroll
+    for (int kb = 0; kb < 16; ++kb) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = kb * 16 + q * 4 + j;
+            float v = base[col] + xs * bf2f(wbf[kMlpOffW1 + (size_t)626 * kMlpH + col]);
+            a[kb][j] = (short)f2bf(v > 0.f ? v : 0.f);
         }
-        x[row * kMlpIn + c] = f2bf(v);
+    }
+    // layers 2 and 3: per output block nb, 16 MFMAs over K, then bias, ReLU,
+    // bf16, and the dot product with w3 (lane holds rows 4q+j, column nb*16+r)
+    float part[4] = {0.f, 0.f, 0.f, 0.f};
+    const uint2 *fr = (const uint2 *)w2frag;
+#pragma unroll 1
+    for (int nb = 0; nb < 16; ++nb) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 16; ++kb) {
+            const uint2 w = fr[(kb * 16 + nb) * 64 + lane];
+            bf16x4 b;
+            b[0] = (short)(w.x & 0xffffu);
+            b[1] = (short)(w.x >> 16);
+            b[2] = (short)(w.y & 0xffffu);
+            b[3] = (short)(w.y >> 16);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kb], b, acc, 0, 0, 0);
+        }
+        const int col = nb * 16 + r;
+        const float b2 = master[kMlpOffB2 + col];
+        const float w3 = bf2f(wbf[kMlpOffW3 + col]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float z = acc[j] + b2;
+            part[j] += bf2f(f2bf(z > 0.f ? z : 0.f)) * w3;
+        }
+    }
+    // sum over the 16 lanes holding the same rows
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        for (int off = 8; off > 0; off >>= 1) part[j] += __shfl_xor(part[j], off, 16);
+    if (r == 0) {
+        const float b3 = master[kMlpOffB3];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[slot * kPredThr + q * 4 + j] = part[j] + b3;
     }
 }
 
-__global__ void k_scatter_pred(const float *y, long r0, long nrows, float *pred_inst) {
-    long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row < nrows) pred_inst[r0 + row] = y[row];
+static hipError_t predict_launch(MlpState &m, int width, const Item *items, int stride, const unsigned *count,
+                                 int n_groups, const Item *roots, const int *thr0, float *out, hipStream_t s) {
+    hipError_t e = hipSuccess;
+    if (!m.w2frag && (e = hipMalloc(&m.w2frag, (size_t)16 * 16 * 64 * 4 * 2)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_w2frag, dim3(64), dim3(256), 0, s, m.wbf, m.w2frag);
+    const long waves = (long)n_groups * stride;
+    dim3 grid((unsigned)((waves + 3) / 4));
+    if (width == 4)
+        hipLaunchKernelGGL((k_predict16<4>), grid, dim3(256), 0, s, items, stride, count, n_groups, roots, thr0, m.wbf,
+                           m.master, m.w2frag, out);
+    else
+        hipLaunchKernelGGL((k_predict16<5>), grid, dim3(256), 0, s, items, stride, count, n_groups, roots, thr0, m.wbf,
+                           m.master, m.w2frag, out);
+    return hipGetLastError();
 }
 
 hipError_t mlp_predict_frontier(MlpState &m, int width, const Item *frontier, const unsigned *nfront,
-                                const Item *roots, int n_inst, int capl, int K, float *pred, hipStream_t s) {
-    std::vector<unsigned> nf((size_t)n_inst);
-    std::vector<Item> rt((size_t)n_inst);
-    hipError_t e = hipMemcpyAsync(nf.data(), nfront, nf.size() * 4, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(rt.data(), roots, rt.size() * sizeof(Item), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return e;
-    const long chunk = 1 << 16;
-    if ((e = mlp_reserve(m, (size_t)chunk)) != hipSuccess) return e;
-    for (int inst = 0; inst < n_inst; ++inst) {
-        const unsigned n = nf[inst] < (unsigned)capl ? nf[inst] : (unsigned)capl;
-        const long rows = (long)n * K;
-        const int L0 = meta_h(rt[inst].meta);
-        const Item *F = frontier + (size_t)inst * capl;
-        float *P = pred + (size_t)inst * capl * K;
-        for (long r0 = 0; r0 < rows; r0 += chunk) {
-            const long nr = rows - r0 < chunk ? rows - r0 : chunk;
-            dim3 g((unsigned)((nr + 3) / 4));
-            if (width == 4)
-                hipLaunchKernelGGL((k_features_frontier<4>), g, dim3(256), 0, s, F, n, K, L0, r0, nr, m.x);
-            else
-                hipLaunchKernelGGL((k_features_frontier<5>), g, dim3(256), 0, s, F, n, K, L0, r0, nr, m.x);
-            if ((e = mlp_forward(m, (int)nr, s)) != hipSuccess) return e;
-            hipLaunchKernelGGL(k_scatter_pred, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, m.y, r0, nr, P);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
-        }
-    }
-    return hipSuccess;
+                                const Item *roots, int n_inst, int capl, float *pred, hipStream_t s) {
+    return predict_launch(m, width, frontier, capl, nfront, n_inst, roots, nullptr, pred, s);
+}
+
+hipError_t mlp_predict_items(MlpState &m, int width, const Item *items, const int *thr0, int n, float *out,
+                             hipStream_t s) {
+    return predict_launch(m, width, items, n, nullptr, 1, nullptr, thr0, out, s);
 }
 
 } // namespace spz

@@ -76,7 +76,7 @@ EXPORTS = ["spz_default_config", "spz_create", "spz_build_pdb", "spz_export_pdb_
            "spz_gen_random", "spz_gather_probe", "spz_mlp_param_count", "spz_mlp_init",
            "spz_mlp_set_params", "spz_mlp_get_params", "spz_mlp_forward", "spz_mlp_grad",
            "spz_mlp_train", "spz_get_samples", "spz_reconfigure", "spz_get_config",
-           "spz_autotune"]
+           "spz_autotune", "spz_mlp_predict"]
 
 
 def lib() -> ctypes.CDLL:
@@ -112,6 +112,7 @@ def lib() -> ctypes.CDLL:
     L.spz_mlp_set_params.argtypes = [vp, f32p]
     L.spz_mlp_get_params.argtypes = [vp, f32p]
     L.spz_mlp_forward.argtypes = [vp, u8p, i32p, ctypes.c_int32, f32p]
+    L.spz_mlp_predict.argtypes = [vp, u8p, i32p, ctypes.c_int32, f32p]
     L.spz_mlp_grad.argtypes = [vp, u8p, i32p, f32p, ctypes.c_int32, f32p, f32p]
     L.spz_mlp_train.argtypes = [vp, u8p, i32p, f32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                 ctypes.c_float, ctypes.c_uint64, f32p]
@@ -283,6 +284,16 @@ class Solver:
         self._check(lib().spz_mlp_forward(self._ctx, _u8p(b), tp, len(b),
                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
                     "spz_mlp_forward")
+        return out
+
+    def mlp_predict(self, boards, thr0) -> np.ndarray:
+        """Predictions of the learned load balancer's fused kernel: row i
+        holds boards[i] at thresholds thr0[i], thr0[i] + 2, ... (16)."""
+        b, t, tp = self._rows(boards, thr0)
+        out = np.zeros((len(b), 16), dtype=np.float32)
+        self._check(lib().spz_mlp_predict(self._ctx, _u8p(b), tp, len(b),
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "spz_mlp_predict")
         return out
 
     def mlp_grad(self, boards, thr, target):

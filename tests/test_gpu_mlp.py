@@ -49,6 +49,39 @@ def test_forward_matches_reference(pdbs):
     assert np.max(np.abs(y - yr)) <= FWD_ATOL * max(1.0, float(np.max(np.abs(yr))))
 
 
+def test_fused_prediction_matches_forward(pdbs):
+    """The learned balancer's fused kernel (layer 1 as one-hot row sums plus a
+    rank-1 threshold term, layers 2-3 in one MFMA pass per node) against the
+    plain forward pass and the fp64 restatement, at all 16 thresholds."""
+    pdbs.mlp_init(5)
+    p = pdbs.mlp_get_params()
+    b, thr, h = rows(pdbs, 96, 31)
+    pred = pdbs.mlp_predict(b, thr)
+    assert pred.shape == (96, 16)
+    rb = np.repeat(b, 16, axis=0)
+    rt = (np.repeat(thr, 16) + 2 * np.tile(np.arange(16), 96)).astype(np.int32)
+    fwd = pdbs.mlp_forward(rb, rt).reshape(96, 16)
+    yr, _ = M.forward(p, M.features(rb, rt, np.repeat(h, 16)))
+    scale = max(1.0, float(np.max(np.abs(yr))))
+    assert np.max(np.abs(pred - fwd)) <= FWD_ATOL * scale
+    assert np.max(np.abs(pred - yr.reshape(96, 16))) <= FWD_ATOL * scale
+
+
+def test_fused_prediction_15_puzzle():
+    boards = np.stack([O.walk(4, 50, 9_100_000 + i) for i in range(40)])
+    with spz.Solver(width=4) as s:
+        s.mlp_init(3)
+        p = s.mlp_get_params()
+        h = s.eval_h(boards)
+        thr = (h + 2).astype(np.int32)
+        pred = s.mlp_predict(boards, thr)
+        rb = np.repeat(boards, 16, axis=0)
+        rt = (np.repeat(thr, 16) + 2 * np.tile(np.arange(16), 40)).astype(np.int32)
+        yr, _ = M.forward(p, M.features(rb, rt, np.repeat(h, 16)))
+    scale = max(1.0, float(np.max(np.abs(yr))))
+    assert np.max(np.abs(pred.reshape(-1) - yr)) <= FWD_ATOL * scale
+
+
 def test_gradient_matches_fp64_reference(pdbs):
     pdbs.mlp_init(9)
     p = pdbs.mlp_get_params()

@@ -38,6 +38,23 @@ def test_graph_replay_matches_eager(golden):
                 assert all(b is None or a == b for a, b in zip(cnt, rec["counts"]))
 
 
+def test_kernel_timing(golden):
+    """time_kernels times every round launch with HIP events (the bench's
+    roofline reads these) and changes no result."""
+    fx = golden("iterations_korf100.json")["instances"][:10]
+    boards = [r["board"] for r in fx]
+    with spz.Solver(width=4) as s:
+        plain = [(r.opt_len, r.counts) for r in s.solve(boards)]
+    with spz.Solver(width=4, time_kernels=True) as s:
+        for rep in range(2):
+            say("timed solve", rep)
+            got = [(r.opt_len, r.counts) for r in s.solve(boards)]
+            st = s.stats()
+            assert got == plain
+            assert st["round_launches"] > 0 and st["round_kernel_ms"] > 0
+            assert st["round_kernel_ms"] <= st["rounds_wall_ms"] * 1.01
+
+
 def test_autotuner_keeps_results(golden):
     fx = golden("iterations_c1_walk40.json")["instances"][:32]
     boards = [r["board"] for r in fx]
