@@ -33,6 +33,7 @@ import spz  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # int32 lane-ops/s: CUs x SIMDs x lanes x clock
+VALU_ISSUE_PEAK = 256 * 2.4e9  # wave64 VALU instructions/s: 4 SIMD16 per CU, 4 cycles each
 
 
 def instances(config: str, n: int | None = None):
@@ -80,14 +81,15 @@ WORKLOAD = {
 }
 
 
-def traffic_per_node(config: str):
-    """Calibrated HBM-side bytes per generated node of k_round, measured by
-    two rocprofv3 --pmc passes on this workload (profiles/ summary)."""
-    path = os.path.join(REPO, "profiles", f"k_round_{config}_pdb_traffic.md")
+def profile_figure(config: str, what: str, key: str):
+    """A per-node figure of k_round measured by rocprofv3 --pmc passes on this
+    workload and kept in its profiles/ summary (what = "traffic": calibrated
+    FETCH_SIZE + WRITE_SIZE bytes; "instructions": VALU wave instructions)."""
+    path = os.path.join(REPO, "profiles", f"k_round_{config}_pdb_{what}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_node:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
 
@@ -295,11 +297,21 @@ def main():
         achieved = nodes_per_launch * 16 / avg_launch_s / 1e12
         roof = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
                 "frac": achieved / VALU_PEAK_TOPS, "traffic": None}
-    tb = traffic_per_node(args.config)
+    tb = profile_figure(args.config, "traffic", "traffic_bytes_per_node")
     if tb is not None:
         roof["traffic"] = tb * nodes_per_launch
-        roof["traffic_source"] = "calibrated FETCH_SIZE+WRITE_SIZE bytes per generated node from " \
-                                 "profiles/k_round_c4_pdb_traffic.md x this run's nodes per launch"
+        roof["traffic_source"] = f"calibrated FETCH_SIZE+WRITE_SIZE bytes per generated node from " \
+                                 f"profiles/k_round_{args.config}_pdb_traffic.md x this run's nodes per launch"
+    vi = profile_figure(args.config, "instructions", "valu_wave_instructions_per_node")
+    if vi is not None:
+        # the limit the counters point to: VALU issue, one wave64 instruction
+        # per CU and cycle
+        rate = vi * nodes_per_launch / avg_launch_s
+        roof["valu_issue"] = {
+            "wave_instructions_per_node": vi, "achieved": rate / 1e9, "peak": VALU_ISSUE_PEAK / 1e9,
+            "unit": "G wave-instructions/s", "frac": rate / VALU_ISSUE_PEAK,
+            "source": f"SQ_INSTS_VALU per generated node from profiles/k_round_{args.config}_pdb_instructions.md "
+                      f"x this run's nodes per launch / its HIP-event launch time"}
     roof["kernel"] = "k_round"
     roof["avg_launch_ms"] = avg_launch_s * 1e3
     roof["launches"] = launches
