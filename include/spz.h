@@ -86,6 +86,8 @@ typedef struct spz_config {
                                 threshold, subtree count) samples per solve
                                 (spz_get_samples); training data of the MLP */
     int64_t queue_items;     /* work-queue capacity in items (0 = auto) */
+    int32_t record_paths;    /* 1: record one optimal solution path per instance
+                                (spz_get_paths); needs max_split_depth <= 32 */
 } spz_config;
 
 typedef struct spz_result {
@@ -182,6 +184,16 @@ int spz_mlp_train(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const
  * width^2), thresholds and subtree counts.  *n_out = number copied; with
  * boards == NULL, the number available. */
 int spz_get_samples(spz_ctx *ctx, uint8_t *boards, int32_t *thr, uint64_t *loads, int32_t max, int32_t *n_out);
+
+/* Solution paths of the last solve (config record_paths): lens[i] = length of
+ * the path recorded for instance i (its optimal length; -1 if none, e.g. not
+ * solved within max_iters or longer than 256 moves); moves[i*max_len + j],
+ * j < min(lens[i], max_len) = the j-th move of the blank: 0 up, 1 right,
+ * 2 left, 3 down -- the direction codes the reference's stack_dump prints
+ * (src_horie/idas_cpu.c:50-57; src_horie/idas_dynamic.cu:1494-1504 prints
+ * them as U R L D).  No reference counterpart for the prefix above a GPU
+ * work item (the reference leaves it out, idas_dynamic.cu:1496-1497). */
+int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens);
 
 /* Statistics of the last solve (stats must hold 24 doubles):
  *  0 rounds run            1 solve wall ms        2 items donated

@@ -22,7 +22,8 @@ hipError_t launch_control(const KParams &p, hipStream_t s);
 hipError_t launch_split_level(int width, int heur, const KParams &p, const PdbGeo *geo, const Item *lvl_in,
                               const unsigned *n_in, Item *lvl_out, unsigned *n_out, unsigned *nfront,
                               unsigned *topgen, int *top_goal, int *frozen, int depth, int max_depth,
-                              unsigned target, hipStream_t s);
+                              unsigned target, const unsigned long long *lp_in, unsigned long long *lp_out,
+                              unsigned long long *fp_out, hipStream_t s);
 hipError_t launch_eval_h(int width, int heur, const KParams &p, const PdbGeo *geo, Item *items, int n,
                          hipStream_t s);
 hipError_t launch_scatter_roots(const Item *roots, Item *lvl, int capl, int n, hipStream_t s);
@@ -88,6 +89,9 @@ struct spz_ctx {
     DBuf frontier, nfront, topgen, top_goal, frozen, lvl[2], nlvl[2];
     // load balancing
     DBuf c_origin, load, samples, nsamp, pred;
+    // solution paths (record_paths)
+    DBuf qpath, c_ipath, fpath, lpath[2], sol, sol_len;
+    int paths_n = 0; // instances of the last solve with recorded paths
     MlpState mlp;
     bool mlp_alloced = false;
 };
@@ -381,7 +385,8 @@ void spz_destroy(spz_ctx *ctx) {
                    &ctx->nfront,      &ctx->topgen,         &ctx->top_goal,       &ctx->frozen,
                    &ctx->lvl[0],      &ctx->lvl[1],         &ctx->nlvl[0],        &ctx->nlvl[1],
                    &ctx->c_origin,    &ctx->load,           &ctx->samples,        &ctx->nsamp,
-                   &ctx->pred};
+                   &ctx->pred,        &ctx->qpath,          &ctx->c_ipath,        &ctx->fpath,
+                   &ctx->lpath[0],    &ctx->lpath[1],       &ctx->sol,            &ctx->sol_len};
     for (DBuf *b : all) b->release();
     if (ctx->mlp_alloced) mlp_free(ctx->mlp);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -642,6 +647,28 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     KParams p = base_params(ctx, n, max_iters, capl);
     p.n_lanes = (int)lanes;
 
+    // solution paths: every queued item carries its moves from the root
+    const bool paths = ctx->cfg.record_paths != 0;
+    ctx->paths_n = 0;
+    if (paths) {
+        if (ctx->cfg.max_split_depth > 32)
+            return fail(ctx, SPZ_E_ARG, "record_paths needs max_split_depth <= 32");
+        HIPCHK(ctx->qpath.ensure(qcap * kPathWords * 4));
+        HIPCHK(ctx->c_ipath.ensure((size_t)kPathWords * lanes * 4));
+        HIPCHK(ctx->fpath.ensure(front_items * 8));
+        for (int k = 0; k < 2; ++k) HIPCHK(ctx->lpath[k].ensure(front_items * 8));
+        HIPCHK(ctx->sol.ensure(N * kPathWords * 4));
+        HIPCHK(ctx->sol_len.ensure(N * 4));
+        HIPCHK(hipMemsetAsync(ctx->lpath[0].p, 0, front_items * 8, s)); // the roots' empty paths
+        HIPCHK(hipMemsetAsync(ctx->sol.p, 0, N * kPathWords * 4, s));
+        HIPCHK(hipMemsetAsync(ctx->sol_len.p, 0xff, N * 4, s)); // -1: none yet
+        p.qpath = ctx->qpath.as<uint32_t>();
+        p.c_ipath = ctx->c_ipath.as<uint32_t>();
+        p.fpath = ctx->fpath.as<unsigned long long>();
+        p.sol = ctx->sol.as<uint32_t>();
+        p.sol_len = ctx->sol_len.as<int>();
+    }
+
     // roots and their heuristic
     std::vector<Item> roots(N);
     for (int i = 0; i < n; ++i) pack_root(w, boards + (size_t)i * w * w, i, &roots[i]);
@@ -671,7 +698,10 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
                                   ctx->nlvl[cur].as<unsigned>(), ctx->lvl[cur ^ 1].as<Item>(),
                                   ctx->nlvl[cur ^ 1].as<unsigned>(), ctx->nfront.as<unsigned>(),
                                   ctx->topgen.as<unsigned>(), ctx->top_goal.as<int>(), ctx->frozen.as<int>(),
-                                  depth, max_depth, target, s));
+                                  depth, max_depth, target,
+                                  paths ? ctx->lpath[cur].as<unsigned long long>() : nullptr,
+                                  paths ? ctx->lpath[cur ^ 1].as<unsigned long long>() : nullptr,
+                                  paths ? ctx->fpath.as<unsigned long long>() : nullptr, s));
         cur ^= 1;
     }
 
@@ -794,6 +824,10 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
             rc = fail(ctx, SPZ_E_HIP, "search rounds: %s", hipGetErrorString(e));
             break;
         }
+        if (h_ctl[3]) {
+            rc = fail(ctx, SPZ_E_CAPACITY, "work queue overflow: raise queue_items (now %zu items)", ctx->qcap);
+            break;
+        }
         if (h_ctl[1] == 0) break;
         // no lane busy at the end of the last round and nothing queued (a
         // finished iteration has queued its next one by now): no work can
@@ -848,12 +882,30 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
             for (int k = 0; k < max_iters; ++k) iter_counts[(size_t)i * max_iters + k] = ic[(size_t)i * max_iters + k];
     }
     ctx->stats[11] = (double)n;
+    ctx->paths_n = paths ? n : 0;
     if (p.prof) {
         unsigned long long pf[4];
         HIPCHK(hipMemcpy(pf, p.prof, sizeof pf, hipMemcpyDeviceToHost));
         for (int k = 0; k < 4; ++k) ctx->stats[16 + k] = (double)pf[k];
     }
     ctx->stats[12] = all_nodes;
+    return SPZ_OK;
+}
+
+int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens) {
+    if (!ctx || !lens || max_len < 0 || (max_len > 0 && !moves)) return SPZ_E_ARG;
+    if (ctx->paths_n <= 0) return fail(ctx, SPZ_E_ARG, "the last solve recorded no paths (config record_paths)");
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    const size_t n = (size_t)ctx->paths_n;
+    std::vector<int> len(n);
+    std::vector<uint32_t> w(n * kPathWords);
+    HIPCHK(hipMemcpy(len.data(), ctx->sol_len.p, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(w.data(), ctx->sol.p, n * kPathWords * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+        lens[i] = len[i];
+        for (int j = 0; j < max_len; ++j)
+            moves[i * max_len + j] = j < len[i] ? (uint8_t)((w[i * kPathWords + j / 16] >> (2 * (j % 16))) & 3u) : 0;
+    }
     return SPZ_OK;
 }
 

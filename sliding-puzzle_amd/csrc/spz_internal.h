@@ -41,6 +41,8 @@ struct Sample {
     unsigned long long load;
 };
 
+constexpr int kPathWords = 16;            // path storage: 16 words of 16 moves
+constexpr int kPathMax = 16 * kPathWords; // longest recordable path (256 moves)
 constexpr int kPredK = 16; // thresholds predicted per frontier node
 
 __host__ __device__ inline uint32_t meta_pack(int blank, int g, int h, int prev) {
@@ -72,7 +74,7 @@ struct KParams {
     int *opt;
     unsigned long long *iter_counts; // [n_inst][max_iters]
     int *restart_list;
-    unsigned int *ctl;           // 0 n_restart, 1 active, 2 busy, 3 error,
+    unsigned int *ctl;           // 0 n_restart, 1 active, 2 busy, 3 queue overflow,
                                  // 4 donate_per_lane, 5 budget, 6 rounds,
                                  // 7 donated, 8 overflow exports, 9/10 big/small
                                  // budget, 12 busy at last round end
@@ -111,6 +113,14 @@ struct KParams {
     // pattern databases: the two canonical tables in direct 25^6 addressing,
     // laid end to end (table 1 at kPdbDirect)
     const uint8_t *pdb;
+
+    // solution paths (record_paths; all null when off): moves of the blank
+    // from the instance root, 2 bits each, kPathWords words per path
+    uint32_t *qpath;                 // [qcap][kPathWords] paths of the queued items
+    uint32_t *c_ipath;               // [kPathWords][n_lanes] path of each lane's item root
+    const unsigned long long *fpath; // [n_inst][capl] paths of the frontier nodes
+    uint32_t *sol;                   // [n_inst][kPathWords] one optimal path per instance
+    int *sol_len;                    // [n_inst] its length, -1 until recorded
 };
 
 } // namespace spz

@@ -77,6 +77,46 @@ __device__ __forceinline__ void put_item(const KParams &p, unsigned long long po
     p.q[pos & p.qmask] = it;
 }
 
+// Reserve k slots at the queue tail.  Items in [min(head, limit), tail) are
+// not taken yet; a tail more than the ring's capacity past them would
+// overwrite some, so that sets the error word (ctl[3]) and the host fails the
+// solve with SPZ_E_CAPACITY rather than return counts from a corrupt queue.
+// head is read as it stands, possibly behind, which only makes the test
+// stricter.
+__device__ __forceinline__ unsigned long long q_reserve(const KParams &p, unsigned long long k) {
+    const unsigned long long pos = atomicAdd(&p.q_ctl[2], k);
+    const unsigned long long h = *(volatile unsigned long long *)&p.q_ctl[0], lim = p.q_ctl[1];
+    const unsigned long long live_from = h < lim ? h : lim;
+    if (pos + k - live_from > p.qmask + 1ull) atomicOr(&p.ctl[3], 1u);
+    return pos;
+}
+
+// Solution paths: move i of a path sits in bits 2(i%16) of word i/16; a
+// path's words are `stride` apart (1 in the queue and solution arrays, the
+// lane count in the per-lane array).  Bits past a path's length are zero, so
+// a copied prefix needs no clearing.  The loops stay rolled: this code runs
+// rarely, and unrolled it would hold a whole path in registers and cost the
+// search kernel occupancy.
+// Paths are addressed as base[off + word * stride] with 32-bit offsets
+// (every path array holds fewer than 2^32 words), so the base stays scalar.
+__device__ __forceinline__ void path_put(uint32_t *base, uint32_t off, uint32_t stride, int i, int d) {
+    if (i >= kPathMax) return; // too long to record; no goal through it is recorded
+    uint32_t &x = base[off + (uint32_t)(i >> 4) * stride];
+    const int sh = 2 * (i & 15);
+    x = (x & ~(3u << sh)) | ((uint32_t)d << sh);
+}
+__device__ __forceinline__ void path_copy(const uint32_t *sb, uint32_t so, uint32_t ss, uint32_t *db, uint32_t dof,
+                                          uint32_t dstr) {
+#pragma unroll 1
+    for (uint32_t k = 0; k < (uint32_t)kPathWords; ++k) db[dof + k * dstr] = sb[so + k * ss];
+}
+__device__ __forceinline__ void path_from_u64(unsigned long long v, uint32_t *db, uint32_t dof) {
+    db[dof] = (uint32_t)v;
+    db[dof + 1] = (uint32_t)(v >> 32);
+#pragma unroll 1
+    for (uint32_t k = 2; k < (uint32_t)kPathWords; ++k) db[dof + k] = 0u;
+}
+
 template <int W, int HEUR, class BD>
 __device__ __forceinline__ void pos_from_item(Pos<W, HEUR, BD> &s, const uint32_t (&b)[4], uint32_t meta,
                                               const PdbCtx &P) {
@@ -108,8 +148,11 @@ __device__ __forceinline__ Pos<W, HEUR> reg_pos(const Pos<W, HEUR, BD> &s) {
 // across rounds in the c_* context arrays.  The stack holds one byte per
 // level: the move taken from that level (2 bits) and the moves still to
 // try there (4-bit mask of children that passed the f-test).
+// The 15-puzzle kernel is held to 64 VGPRs (8 waves per SIMD: its LDS use is
+// small, and it fits without scratch); the 24-puzzle kernels are bounded by
+// their LDS (stack and boards) at 3 waves per SIMD.
 template <int W, int HEUR>
-__global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
+__global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, const PdbGeo *geo_g) {
     const int MAXD = p.maxd;
     extern __shared__ uint8_t lds[];
     __shared__ PdbGeo geo_s;
@@ -224,6 +267,15 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
         prev = d;
         st = ST_FRESH;
     };
+    // path of a new queue item: the item root's path, the moves of stack
+    // levels 0..nst-1, then move last_d
+    auto item_path = [&](unsigned long long qpos, int nst, int last_d) {
+        const uint32_t qo = (uint32_t)(qpos & p.qmask) * kPathWords;
+        path_copy(p.c_ipath, (uint32_t)gl, (uint32_t)NL, p.qpath, qo, 1);
+        const int gr = meta_g(rmeta);
+        for (int k = 0; k < nst; ++k) path_put(p.qpath, qo, 1, gr + k, SE::dir(stk[k * sstride]));
+        path_put(p.qpath, qo, 1, gr + nst, last_d);
+    };
 
     uint32_t pf_idle = 0, pf_exp = 0, pf_pop = 0, pf_steps = 0;
     for (int step = 0; step < budget; ++step) {
@@ -238,10 +290,14 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                 base = shfl_u64(base, leader);
                 if (st == ST_IDLE) {
                     unsigned long long my = base + __popcll(need & lanemask_lt());
-                    if (my < limit)
+                    if (my < limit) {
                         start_item(p.q[my & p.qmask]);
-                    else
+                        if (p.qpath)
+                            path_copy(p.qpath, (uint32_t)(my & p.qmask) * kPathWords, 1, p.c_ipath, (uint32_t)gl,
+                                      (uint32_t)NL);
+                    } else {
                         st = ST_DRAINED;
+                    }
                 }
             }
         }
@@ -312,6 +368,15 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
             bool xp = true;
             if (h == 0) {
                 atomicMin(&p.found[inst], g);
+                // the first goal of the instance records its path (every goal
+                // found has the optimal depth: the first iteration with a goal
+                // has threshold = optimum)
+                if (p.sol && g <= kPathMax && atomicCAS(&p.sol_len[inst], -1, g) == -1) {
+                    const uint32_t so = (uint32_t)inst * kPathWords;
+                    path_copy(p.c_ipath, (uint32_t)gl, (uint32_t)NL, p.sol, so, 1);
+                    const int gr = meta_g(rmeta);
+                    for (int k = 0; k < sp; ++k) path_put(p.sol, so, 1, gr + k, SE::dir(stk[k * sstride]));
+                }
                 if (mode == 1) {
                     st = ST_POP;
                     xp = false;
@@ -385,11 +450,12 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
                     // stack full: the passing children become queue items
                     int k = __builtin_popcount(pass);
                     atomicAdd(&p.pend[inst], (unsigned)k);
-                    unsigned long long pos = atomicAdd(&p.q_ctl[2], (unsigned long long)k);
+                    unsigned long long pos = q_reserve(p, (unsigned long long)k);
                     atomicAdd(&p.ctl[8], (unsigned)k);
                     while (pass) {
                         int d = __builtin_ctz(pass);
                         pass &= pass - 1;
+                        if (p.qpath) item_path(pos, sp, d);
                         Pos<W, HEUR> c = reg_pos(s);
                         do_move<W, HEUR>(c, d, P);
                         put_item<W, HEUR>(p, pos++, c, g + 1, d, inst, origin);
@@ -426,13 +492,29 @@ __global__ void __launch_bounds__(256) k_round(KParams p, const PdbGeo *geo_g) {
         int total = __shfl(incl, 63);
         unsigned long long base = 0;
         if (lane == 0) {
-            base = atomicAdd(&p.q_ctl[2], (unsigned long long)total);
+            base = q_reserve(p, (unsigned long long)total);
             atomicAdd(&p.ctl[7], (unsigned)total);
         }
         base = shfl_u64(base, 0);
         if (want > 0) {
             unsigned long long pos = base + (unsigned long long)(incl - want);
             atomicAdd(&p.pend[inst], (unsigned)want);
+            if (p.qpath) {
+                // the donated items' paths first, in the order of the pass
+                // below (a pass of its own keeps the replayed positions out
+                // of its registers)
+                unsigned long long pp = pos;
+                int emitted = 0;
+                for (int k = 0; k < sp && emitted < want; ++k) {
+                    uint32_t R = SE::rest(stk[k * sstride]);
+                    while (R && emitted < want) {
+                        const int d2 = __builtin_ctz(R);
+                        R &= R - 1;
+                        item_path(pp++, k, d2);
+                        ++emitted;
+                    }
+                }
+            }
             // replay from the item root along the taken moves
             Pos<W, HEUR> c;
             pos_from_item<W, HEUR>(c, root, rmeta, P);
@@ -598,7 +680,7 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
                 off += n;
             }
             s_n = off;
-            s_base = off ? atomicAdd(&p.q_ctl[2], (unsigned long long)off) : 0ull;
+            s_base = off ? q_reserve(p, (unsigned long long)off) : 0ull;
             if (p.top_goal[inst] <= L) atomicMin(&p.found[inst], p.top_goal[inst]);
             p.cnt[inst] = s_cnt;
             p.pend[inst] = off;
@@ -610,6 +692,9 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
                 unsigned k = atomicAdd(&s_bucket[lb_bucket(p, inst, i, it)], 1u);
                 x.x1 = i;
                 p.q[(s_base + k) & p.qmask] = x;
+                if (p.qpath)
+                    path_from_u64(p.fpath[(size_t)inst * p.capl + i], p.qpath,
+                                  (uint32_t)((s_base + k) & p.qmask) * kPathWords);
             }
         }
         __syncthreads();
@@ -681,7 +766,9 @@ template <int W, int HEUR>
 __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *geo_g, const Item *lvl_in,
                                                      const unsigned *n_in, Item *lvl_out, unsigned *n_out,
                                                      unsigned *nfront, unsigned *topgen, int *top_goal,
-                                                     int *frozen, int depth, int max_depth, unsigned target) {
+                                                     int *frozen, int depth, int max_depth, unsigned target,
+                                                     const unsigned long long *lp_in, unsigned long long *lp_out,
+                                                     unsigned long long *fp_out) {
     __shared__ PdbGeo geo_s;
     PdbCtx P{&geo_s, p.pdb};
     if constexpr (HEUR == 1) {
@@ -701,7 +788,10 @@ __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *ge
     const unsigned n = min(n_in[inst], (unsigned)p.capl);
     const bool freeze = (n >= target) || (depth >= max_depth);
     if (freeze) {
-        if (j < n) ((Item *)p.frontier)[(size_t)inst * p.capl + j] = lvl_in[(size_t)inst * p.capl + j];
+        if (j < n) {
+            ((Item *)p.frontier)[(size_t)inst * p.capl + j] = lvl_in[(size_t)inst * p.capl + j];
+            if (fp_out) fp_out[(size_t)inst * p.capl + j] = lp_in[(size_t)inst * p.capl + j];
+        }
         if (j == 0) {
             nfront[inst] = n;
             frozen[inst] = depth + 1;
@@ -719,7 +809,12 @@ __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *ge
     // or not
     int f = g + s.hs.h;
     if ((int)it.x0 > f) f = (int)it.x0;
-    if (s.hs.h == 0) atomicMin(&top_goal[inst], f);
+    if (s.hs.h == 0) {
+        atomicMin(&top_goal[inst], f);
+        // levels run in depth order, so the first goal recorded is a shallowest one
+        if (p.sol && atomicCAS(&p.sol_len[inst], -1, g) == -1)
+            path_from_u64(lp_in[(size_t)inst * p.capl + j], p.sol, (uint32_t)inst * kPathWords);
+    }
     uint32_t legal = (br > 0 ? 1u : 0u) | (bc < W - 1 ? 2u : 0u) | (bc > 0 ? 4u : 0u) | (br < W - 1 ? 8u : 0u);
     uint32_t gen = legal & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
     atomicAdd(&topgen[(size_t)inst * kFBins + (f < kFBins ? f : kFBins - 1)], (unsigned)__builtin_popcount(gen));
@@ -735,7 +830,11 @@ __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *ge
         ci.inst = (uint32_t)inst;
         ci.x0 = (uint32_t)f;
         ci.x1 = 0;
-        if (k < (unsigned)p.capl) lvl_out[(size_t)inst * p.capl + k] = ci;
+        if (k < (unsigned)p.capl) {
+            lvl_out[(size_t)inst * p.capl + k] = ci;
+            // split depth <= 32 when paths are recorded: move g fits the u64
+            if (lp_out) lp_out[(size_t)inst * p.capl + k] = lp_in[(size_t)inst * p.capl + j] | ((unsigned long long)d << (2 * g));
+        }
         ++k;
     }
 }
@@ -818,18 +917,19 @@ hipError_t launch_control(const KParams &p, hipStream_t s) {
 hipError_t launch_split_level(int width, int heur, const KParams &p, const PdbGeo *geo, const Item *lvl_in,
                               const unsigned *n_in, Item *lvl_out, unsigned *n_out, unsigned *nfront,
                               unsigned *topgen, int *top_goal, int *frozen, int depth, int max_depth,
-                              unsigned target, hipStream_t s) {
+                              unsigned target, const unsigned long long *lp_in, unsigned long long *lp_out,
+                              unsigned long long *fp_out, hipStream_t s) {
     size_t total = (size_t)p.n_inst * p.capl;
     dim3 grid((unsigned)((total + 255) / 256));
     if (width == 4)
         hipLaunchKernelGGL((k_split_level<4, 0>), grid, dim3(256), 0, s, p, geo, lvl_in, n_in, lvl_out, n_out, nfront,
-                           topgen, top_goal, frozen, depth, max_depth, target);
+                           topgen, top_goal, frozen, depth, max_depth, target, lp_in, lp_out, fp_out);
     else if (heur == 0)
         hipLaunchKernelGGL((k_split_level<5, 0>), grid, dim3(256), 0, s, p, geo, lvl_in, n_in, lvl_out, n_out, nfront,
-                           topgen, top_goal, frozen, depth, max_depth, target);
+                           topgen, top_goal, frozen, depth, max_depth, target, lp_in, lp_out, fp_out);
     else
         hipLaunchKernelGGL((k_split_level<5, 1>), grid, dim3(256), 0, s, p, geo, lvl_in, n_in, lvl_out, n_out, nfront,
-                           topgen, top_goal, frozen, depth, max_depth, target);
+                           topgen, top_goal, frozen, depth, max_depth, target, lp_in, lp_out, fp_out);
     return hipGetLastError();
 }
 

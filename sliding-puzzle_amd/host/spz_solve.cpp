@@ -24,6 +24,8 @@ static void usage() {
                  "  -m all|first           all solutions of the last iteration (default) or stop at a goal\n"
                  "  --batch                solve all files in one batch (default: one per launch)\n"
                  "  --iters                print per-iteration counts\n"
+                 "  --path                 print an optimal solution (\"len=N: d d ...\", blank moves\n"
+                 "                         0 up, 1 right, 2 left, 3 down)\n"
                  "  --csv FILE             write per-instance seconds\n"
                  "  --pdb-cache FILE       read the PDB tables from FILE, or write them there\n"
                  "  --export-tab DIR       write pat24.1256712.tab and pat24.34891314.tab\n"
@@ -50,7 +52,7 @@ int main(int argc, char **argv) {
     spz_config cfg;
     int heur = SPZ_H_MANHATTAN, mode = SPZ_MODE_ALL, lb = SPZ_LB_UNIFORM;
     int frontier = 2048, wg = 256, depth = 0, budget = 0, max_iters = 64;
-    bool batch = false, iters = false, autotune = false;
+    bool batch = false, iters = false, autotune = false, path = false;
     std::string csv, cache, tabdir;
     std::vector<std::string> files;
     for (int i = 1; i < argc; ++i) {
@@ -71,6 +73,8 @@ int main(int argc, char **argv) {
             batch = true;
         } else if (a == "--iters") {
             iters = true;
+        } else if (a == "--path") {
+            path = true;
         } else if (a == "--csv") {
             csv = next();
         } else if (a == "--pdb-cache") {
@@ -121,6 +125,7 @@ int main(int argc, char **argv) {
     cfg.lds_stack_depth = depth;
     cfg.round_budget = budget;
     cfg.lb_policy = lb;
+    cfg.record_paths = path ? 1 : 0;
     spz_ctx *ctx = nullptr;
     int rc = spz_create(&ctx, &cfg);
     if (rc != SPZ_OK) {
@@ -157,17 +162,22 @@ int main(int argc, char **argv) {
     std::vector<spz_result> res((size_t)n);
     std::vector<uint64_t> ic((size_t)n * max_iters);
     std::vector<double> secs((size_t)n);
+    const int max_len = 256;
+    std::vector<uint8_t> moves((size_t)n * max_len);
+    std::vector<int32_t> lens((size_t)n, -1);
     if (batch) {
         auto t0 = std::chrono::steady_clock::now();
         rc = spz_solve_batch(ctx, boards.data(), n, max_iters, res.data(), ic.data());
         double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         for (auto &x : secs) x = s / n;
+        if (rc == SPZ_OK && path) rc = spz_get_paths(ctx, moves.data(), max_len, lens.data());
     } else {
         for (int i = 0; i < n && rc == SPZ_OK; ++i) {
             auto t0 = std::chrono::steady_clock::now();
             rc = spz_solve_batch(ctx, boards.data() + (size_t)i * n2, 1, max_iters, &res[i],
                                  ic.data() + (size_t)i * max_iters);
             secs[i] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (rc == SPZ_OK && path) rc = spz_get_paths(ctx, moves.data() + (size_t)i * max_len, max_len, &lens[i]);
         }
     }
     if (rc != SPZ_OK) {
@@ -182,6 +192,12 @@ int main(int argc, char **argv) {
                 std::printf("f_limit=%3d, expanded nodes = %llu\n", res[i].h0 + 2 * k,
                             (unsigned long long)ic[(size_t)i * max_iters + k]);
         std::printf("%s %d %llu\n", files[i].c_str(), res[i].opt_len, (unsigned long long)res[i].total_nodes);
+        if (path && lens[i] >= 0) {
+            // the reference CPU solver's stack_dump line (src_horie/idas_cpu.c:50-57)
+            std::printf("len=%d: ", lens[i]);
+            for (int j = 0; j < lens[i] && j < max_len; ++j) std::printf("%d ", moves[(size_t)i * max_len + j]);
+            std::printf("\n");
+        }
         if (cf) std::fprintf(cf, "%f\n", secs[i]);
     }
     if (cf) std::fclose(cf);

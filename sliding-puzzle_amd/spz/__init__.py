@@ -47,7 +47,7 @@ class Config(ctypes.Structure):
                 ("lb_policy", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("time_kernels", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("log_samples", ctypes.c_int32),
-                ("queue_items", ctypes.c_int64)]
+                ("queue_items", ctypes.c_int64), ("record_paths", ctypes.c_int32)]
 
 
 class _Result(ctypes.Structure):
@@ -76,7 +76,7 @@ EXPORTS = ["spz_default_config", "spz_create", "spz_build_pdb", "spz_export_pdb_
            "spz_gen_random", "spz_gather_probe", "spz_mlp_param_count", "spz_mlp_init",
            "spz_mlp_set_params", "spz_mlp_get_params", "spz_mlp_forward", "spz_mlp_grad",
            "spz_mlp_train", "spz_get_samples", "spz_reconfigure", "spz_get_config",
-           "spz_autotune", "spz_mlp_predict"]
+           "spz_autotune", "spz_mlp_predict", "spz_get_paths"]
 
 
 def lib() -> ctypes.CDLL:
@@ -113,6 +113,7 @@ def lib() -> ctypes.CDLL:
     L.spz_mlp_get_params.argtypes = [vp, f32p]
     L.spz_mlp_forward.argtypes = [vp, u8p, i32p, ctypes.c_int32, f32p]
     L.spz_mlp_predict.argtypes = [vp, u8p, i32p, ctypes.c_int32, f32p]
+    L.spz_get_paths.argtypes = [vp, u8p, ctypes.c_int32, i32p]
     L.spz_mlp_grad.argtypes = [vp, u8p, i32p, f32p, ctypes.c_int32, f32p, f32p]
     L.spz_mlp_train.argtypes = [vp, u8p, i32p, f32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                 ctypes.c_float, ctypes.c_uint64, f32p]
@@ -183,7 +184,7 @@ class Solver:
                  workgroup_size: int = 256, lanes: int = 0, round_budget: int = 0,
                  lb_policy: int = LB_UNIFORM, device: int = 0, queue_items: int = 0,
                  time_kernels: bool = False, lds_stack_depth: int = 0, log_samples: int = 0,
-                 use_graph: bool = True):
+                 use_graph: bool = True, record_paths: bool = False):
         L = lib()
         cfg = Config()
         L.spz_default_config(ctypes.byref(cfg), width, heuristic)
@@ -200,6 +201,7 @@ class Solver:
         cfg.lds_stack_depth = lds_stack_depth
         cfg.log_samples = log_samples
         cfg.use_graph = 1 if use_graph else 0
+        cfg.record_paths = 1 if record_paths else 0
         self.cfg = cfg
         self.width = width
         self._ctx = ctypes.c_void_p()
@@ -285,6 +287,18 @@ class Solver:
                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
                     "spz_mlp_forward")
         return out
+
+    def get_paths(self, max_len: int = 256):
+        """One optimal solution per instance of the last solve (record_paths):
+        a list of move lists (0 up, 1 right, 2 left, 3 down: the blank's
+        moves), None where no path was recorded."""
+        n = getattr(self, "_last_n", 0)
+        moves = np.zeros((n, max_len), dtype=np.uint8)
+        lens = np.zeros(n, dtype=np.int32)
+        self._check(lib().spz_get_paths(self._ctx, _u8p(moves), max_len,
+                                        lens.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+                    "spz_get_paths")
+        return [None if ln < 0 else [int(x) for x in moves[i, :ln]] for i, ln in enumerate(lens)]
 
     def mlp_predict(self, boards, thr0) -> np.ndarray:
         """Predictions of the learned load balancer's fused kernel: row i
@@ -373,6 +387,7 @@ class Solver:
         self._check(lib().spz_solve_batch(self._ctx, _u8p(b), n, max_iters, out,
                                           ic.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))),
                     "spz_solve_batch")
+        self._last_n = n
         res = []
         for i in range(n):
             r = out[i]

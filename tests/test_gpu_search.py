@@ -150,3 +150,19 @@ def test_repeat_solves_same_context(golden):
             res = s.solve([r["board"] for r in kx] + [r["board"] for r in recs])
             for rec, r in zip(kx + recs, res):
                 check_counts(rec, r)
+
+
+def test_queue_overflow_reported(golden):
+    """A work queue too small for the batch (1024 items against 20 x 2048
+    frontier nodes) fails the solve with a capacity error instead of
+    returning counts from overwritten items; the same context then solves
+    correctly with the automatic size."""
+    fx = golden("iterations_c1_walk40.json")["instances"][:20]
+    boards = [r["board"] for r in fx]
+    with spz.Solver(width=4, queue_items=1024) as s:
+        with pytest.raises(spz.SpzError) as e:
+            s.solve(boards)
+        assert "queue" in str(e.value)
+        s.reconfigure(queue_items=0)
+        for rec, r in zip(fx, s.solve(boards)):
+            check_counts(rec, r)
