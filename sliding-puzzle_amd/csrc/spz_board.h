@@ -92,24 +92,26 @@ template <> struct Board<5> {
 };
 
 // host helper: board from a row-major byte array
-// 5x5 board of the search kernel, held in LDS: one byte per cell, the
-// cells of a workgroup's lanes interleaved (cell c of lane x at
-// base[c * stride + x]), so reading the tile next to the blank is one LDS
-// load and a slide two byte stores.  Copying it copies the reference, not
-// the cells: regs() gives a register copy.
+// 5x5 board of the search kernel, held in LDS: one byte per cell, each lane
+// its own kLdsBoardBytes (7-dword) region, so reading the tile next to the
+// blank is one LDS load and a slide two byte stores.  With an odd dword
+// stride the lanes of a wave start in different banks (bytes of several
+// lanes interleaved in one dword would make lanes reading different cells
+// collide in one bank).  Copying it copies the reference, not the cells:
+// regs() gives a register copy.
+constexpr int kLdsBoardBytes = 28;
 struct LdsBoard5 {
     uint8_t *p;  // this lane's cell 0
-    int stride;  // lanes per workgroup
-    __device__ __forceinline__ int tile(int c) const { return (int)p[c * stride]; }
+    __device__ __forceinline__ int tile(int c) const { return (int)p[c]; }
     __device__ __forceinline__ void slide(int t, int from, int to) {
-        p[to * stride] = (uint8_t)t;
-        p[from * stride] = 0;
+        p[to] = (uint8_t)t;
+        p[from] = 0;
     }
     __device__ __forceinline__ void unpack(const uint32_t (&o)[4]) {
         Board<5> b;
         b.unpack(o);
 #pragma unroll
-        for (int c = 0; c < 25; ++c) p[c * stride] = (uint8_t)b.tile(c);
+        for (int c = 0; c < 25; ++c) p[c] = (uint8_t)b.tile(c);
     }
     __device__ __forceinline__ Board<5> regs() const {
         Board<5> b;
@@ -152,11 +154,11 @@ struct PdbGeo {
     uint8_t pad[2];
     uint32_t reg_mul[25]; // 25^(5-slot) of tile t in reg_k
     uint32_t ref_mul[25];
-    // index weight of tile t on cell c in its regular / reflected lookup,
-    // map_k(c) * 25^(5-slot) (< 2^28), with k (0..3) in bits 28-29: a move's
-    // index change is w[t][to] - w[t][from] (the k bits cancel)
-    uint32_t wr[25][25];
-    uint32_t wf[25][25];
+    // index weights of tile t on cell c in its regular (.x) and reflected (.y)
+    // lookup, map_k(c) * 25^(5-slot) (< 2^28), with k (0..3) in bits 28-29: a
+    // move's index change is w[t][to] - w[t][from] (the k bits cancel).  The
+    // pair is one 8-byte LDS read.
+    uint2 w[25][25];
 };
 constexpr uint32_t kWeightMask = 0x0fffffffu;
 
@@ -215,10 +217,10 @@ __device__ __forceinline__ int pdb_read(const PdbCtx &P, uint32_t idx) { return 
 __device__ __forceinline__ int pdb_child_h(const PdbCtx &P, const HState<1> &s, int t, int a, int b,
                                            uint32_t &vals) {
     const PdbGeo &G = *P.geo;
-    const uint32_t ra = G.wr[t][a], fa = G.wf[t][a];
-    const int kr = (int)(ra >> 28), kf = (int)(fa >> 28);
-    uint32_t ir = sel4(s.i0, s.i1, s.i2, s.i3, kr) + (G.wr[t][b] - ra);
-    uint32_t iff = sel4(s.j0, s.j1, s.j2, s.j3, kf) + (G.wf[t][b] - fa);
+    const uint2 wa = G.w[t][a], wb = G.w[t][b];
+    const int kr = (int)(wa.x >> 28), kf = (int)(wa.y >> 28);
+    uint32_t ir = sel4(s.i0, s.i1, s.i2, s.i3, kr) + (wb.x - wa.x);
+    uint32_t iff = sel4(s.j0, s.j1, s.j2, s.j3, kf) + (wb.y - wa.y);
     int vr = pdb_read(P, ir);
     int vf = pdb_read(P, iff);
     vals = (uint32_t)vr | ((uint32_t)vf << 8);
@@ -231,9 +233,9 @@ __device__ __forceinline__ int pdb_child_h(const PdbCtx &P, const HState<1> &s, 
 __device__ __forceinline__ void pdb_apply_known(const PdbCtx &P, HState<1> &s, int t, int a, int b,
                                                 uint32_t vals) {
     const PdbGeo &G = *P.geo;
-    const uint32_t ra = G.wr[t][a], fa = G.wf[t][a];
-    const int kr = (int)(ra >> 28), kf = (int)(fa >> 28);
-    const uint32_t dr = G.wr[t][b] - ra, df = G.wf[t][b] - fa;
+    const uint2 wa = G.w[t][a], wb = G.w[t][b];
+    const int kr = (int)(wa.x >> 28), kf = (int)(wa.y >> 28);
+    const uint32_t dr = wb.x - wa.x, df = wb.y - wa.y;
     add4(s.i0, s.i1, s.i2, s.i3, kr, dr);
     add4(s.j0, s.j1, s.j2, s.j3, kf, df);
     int vr = (int)(vals & 0xffu), vf = (int)((vals >> 8) & 0xffu);
@@ -246,9 +248,9 @@ __device__ __forceinline__ void pdb_apply_known(const PdbCtx &P, HState<1> &s, i
 
 __device__ __forceinline__ void pdb_apply(const PdbCtx &P, HState<1> &s, int t, int a, int b) {
     const PdbGeo &G = *P.geo;
-    const uint32_t ra = G.wr[t][a], fa = G.wf[t][a];
-    const int kr = (int)(ra >> 28), kf = (int)(fa >> 28);
-    const uint32_t dr = G.wr[t][b] - ra, df = G.wf[t][b] - fa;
+    const uint2 wa = G.w[t][a], wb = G.w[t][b];
+    const int kr = (int)(wa.x >> 28), kf = (int)(wa.y >> 28);
+    const uint32_t dr = wb.x - wa.x, df = wb.y - wa.y;
     add4(s.i0, s.i1, s.i2, s.i3, kr, dr);
     add4(s.j0, s.j1, s.j2, s.j3, kf, df);
     int vr = pdb_read(P, sel4(s.i0, s.i1, s.i2, s.i3, kr));
@@ -272,9 +274,9 @@ template <class BD> __device__ __forceinline__ void pdb_init(const PdbCtx &P, HS
     for (int c = 0; c < 25; ++c) {
         int t = bd.tile(c);
         if (t != 0) {
-            const uint32_t wr = G.wr[t][c], wf = G.wf[t][c];
-            add4(s.i0, s.i1, s.i2, s.i3, (int)(wr >> 28), wr & kWeightMask);
-            add4(s.j0, s.j1, s.j2, s.j3, (int)(wf >> 28), wf & kWeightMask);
+            const uint2 wc = G.w[t][c];
+            add4(s.i0, s.i1, s.i2, s.i3, (int)(wc.x >> 28), wc.x & kWeightMask);
+            add4(s.j0, s.j1, s.j2, s.j3, (int)(wc.y >> 28), wc.y & kWeightMask);
         }
     }
     int a0 = pdb_read(P, s.i0), a1 = pdb_read(P, s.i1), a2 = pdb_read(P, s.i2), a3 = pdb_read(P, s.i3);
@@ -336,11 +338,11 @@ __device__ __forceinline__ uint32_t move_pdb(Pos<W, 1, BD> &s, int d, const PdbC
     const int from = nr * W + nc, to = s.br * W + s.bc;
     const int t = s.bd.tile(from);
     const PdbGeo &G = *P.geo;
-    const uint32_t ra = G.wr[t][from], fa = G.wf[t][from];
-    const int kr = (int)(ra >> 28), kf = (int)(fa >> 28);
+    const uint2 wa = G.w[t][from], wb = G.w[t][to];
+    const int kr = (int)(wa.x >> 28), kf = (int)(wa.y >> 28);
     HState<1> &h = s.hs;
-    add4(h.i0, h.i1, h.i2, h.i3, kr, G.wr[t][to] - ra);
-    add4(h.j0, h.j1, h.j2, h.j3, kf, G.wf[t][to] - fa);
+    add4(h.i0, h.i1, h.i2, h.i3, kr, wb.x - wa.x);
+    add4(h.j0, h.j1, h.j2, h.j3, kf, wb.y - wa.y);
     const int oldr = byte_of(h.v0, kr), oldf = byte_of(h.v1, kf);
     int vr, vf;
     if (known) {

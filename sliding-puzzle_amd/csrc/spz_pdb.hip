@@ -65,8 +65,8 @@ void pdb_geo_build(PdbGeo *g) {
     }
     for (int t = 1; t < 25; ++t)
         for (int c = 0; c < 25; ++c) {
-            g->wr[t][c] = (uint32_t)g->map[g->reg_k[t]][c] * g->reg_mul[t] | ((uint32_t)g->reg_k[t] << 28);
-            g->wf[t][c] = (uint32_t)g->map[g->ref_k[t]][c] * g->ref_mul[t] | ((uint32_t)(g->ref_k[t] - 4) << 28);
+            g->w[t][c].x = (uint32_t)g->map[g->reg_k[t]][c] * g->reg_mul[t] | ((uint32_t)g->reg_k[t] << 28);
+            g->w[t][c].y = (uint32_t)g->map[g->ref_k[t]][c] * g->ref_mul[t] | ((uint32_t)(g->ref_k[t] - 4) << 28);
         }
 }
 

@@ -185,8 +185,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     // stack levels, then (width 5) the lanes' boards
     Pos<W, HEUR, typename HotBoard<W>::T> s;
     if constexpr (W == 5) {
-        s.bd.p = lds + (size_t)MAXD * sstride * sizeof(ST) + threadIdx.x;
-        s.bd.stride = blockDim.x;
+        s.bd.p = lds + (size_t)MAXD * sstride * sizeof(ST) + threadIdx.x * kLdsBoardBytes;
     } else {
         s.bd.unpack({0u, 0u, 0u, 0u});
     }
@@ -893,9 +892,9 @@ hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo
     return launch_round_t<5, 1>(p, geo, wg, s);
 }
 
-// dynamic LDS of k_round: the stack levels, then (width 5) the 25-cell boards
+// dynamic LDS of k_round: the stack levels, then (width 5) the boards
 size_t round_lds_bytes(int width, int maxd, int wg, int heur) {
-    return (size_t)maxd * wg * (heur == 1 ? 2 : 1) + (width == 5 ? (size_t)25 * wg : 0);
+    return (size_t)maxd * wg * (heur == 1 ? 2 : 1) + (width == 5 ? (size_t)kLdsBoardBytes * wg : 0);
 }
 
 const void *round_kernel_ptr(int width, int heur) {
