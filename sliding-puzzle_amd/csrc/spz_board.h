@@ -363,6 +363,50 @@ __device__ __forceinline__ uint32_t move_pdb(Pos<W, 1, BD> &s, int d, const PdbC
     return (uint32_t)oldr | ((uint32_t)oldf << 8);
 }
 
+// move_pdb in two halves, so that the table reads of a move and those of the
+// f-test that follows it are in flight together: the first half moves the
+// tile and updates the indices, and reads (or takes) the two new values; the
+// second applies them to the sums once the f-test's reads are issued.
+struct PendMove {
+    int kr, kf;  // the lookups the move changed
+    int vr, vf;  // their new values
+};
+template <int W, class BD>
+__device__ __forceinline__ PendMove move_pdb_begin(Pos<W, 1, BD> &s, int d, const PdbCtx &P, bool known, uint32_t kv,
+                                                   uint32_t &ov) {
+    const int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
+    const int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
+    const int from = nr * W + nc, to = s.br * W + s.bc;
+    const int t = s.bd.tile(from);
+    const PdbGeo &G = *P.geo;
+    const uint2 wa = G.w[t][from], wb = G.w[t][to];
+    PendMove m;
+    m.kr = (int)(wa.x >> 28);
+    m.kf = (int)(wa.y >> 28);
+    HState<1> &h = s.hs;
+    add4(h.i0, h.i1, h.i2, h.i3, m.kr, wb.x - wa.x);
+    add4(h.j0, h.j1, h.j2, h.j3, m.kf, wb.y - wa.y);
+    ov = (uint32_t)byte_of(h.v0, m.kr) | ((uint32_t)byte_of(h.v1, m.kf) << 8);
+    if (known) {
+        m.vr = (int)(kv & 0xffu);
+        m.vf = (int)((kv >> 8) & 0xffu);
+    } else {
+        m.vr = pdb_read(P, sel4(h.i0, h.i1, h.i2, h.i3, m.kr));
+        m.vf = pdb_read(P, sel4(h.j0, h.j1, h.j2, h.j3, m.kf));
+    }
+    s.bd.slide(t, from, to);
+    s.br = nr;
+    s.bc = nc;
+    return m;
+}
+__device__ __forceinline__ void move_pdb_finish(HState<1> &h, const PendMove &m) {
+    h.A += m.vr - byte_of(h.v0, m.kr);
+    h.B += m.vf - byte_of(h.v1, m.kf);
+    h.v0 = set_byte(h.v0, m.kr, m.vr);
+    h.v1 = set_byte(h.v1, m.kf, m.vf);
+    h.h = h.A > h.B ? h.A : h.B;
+}
+
 // values, before the move, of the two lookups containing the tile that a
 // blank move in direction d slides (packed regular | reflected << 8)
 template <int W, class BD>

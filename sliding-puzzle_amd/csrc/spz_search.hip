@@ -277,6 +277,10 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     };
 
     uint32_t pf_idle = 0, pf_exp = 0, pf_pop = 0, pf_steps = 0;
+    // the restored context is complete before the loop, so the compiler's
+    // wait tracking does not carry its loads into every iteration (where it
+    // would wait on a move's table reads before issuing the f-test's)
+    __builtin_amdgcn_s_waitcnt(0);
     for (int step = 0; step < budget; ++step) {
         // ---- idle lanes take queue items (one atomic per wave) ----
         {
@@ -341,27 +345,83 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
             known = true;
             kv = (((down >= 2) ? dval2 : dval) >> (16 * (down & 1))) & 0xffffu;
         }
-        // ---- one move down per lane and step, both cases on one path ----
-        if (down >= 0) enter_child(down, R, slot, known, kv);
-        if (finish) {
-            if (cnt) {
-                atomicAdd(&p.cnt[inst], (unsigned long long)cnt);
-                atomicAdd(&p.load[(size_t)inst * p.capl + origin], (unsigned long long)cnt);
+        // ---- one move down per lane and step, both cases on one path.  For
+        // the PDB the move's values are applied after the f-test's table
+        // reads are issued, so both sets of reads are in flight together ----
+        PendMove pm;
+        bool pmv = false;
+        if (down >= 0) {
+            if constexpr (HEUR == 1) {
+                uint32_t ov;
+                pm = move_pdb_begin<W>(s, down, P, known, kv, ov);
+                pmv = true;
+                stk[slot * sstride] = SE::make(down, R, ov);
+                sp = slot + 1;
+                ++g;
+                prev = down;
+                st = ST_FRESH;
+            } else {
+                enter_child(down, R, slot, known, kv);
             }
-            __threadfence();
-            unsigned old = atomicSub(&p.pend[inst], 1u);
-            if (old == 1u) finish_iteration(p, inst);
-            cnt = 0;
-            st = ST_IDLE;
         }
 
-        if (p.prof) {
-            pf_steps += 1;
-            pf_idle += (st == ST_IDLE || st == ST_DRAINED) ? 1u : 0u;
-            pf_exp += (st == ST_FRESH) ? 1u : 0u;
-            pf_pop += (st == ST_POP) ? 1u : 0u;
+        // ---- expand, first half (PDB): the children's lookup indices and
+        // table reads; they need the node's indices, not its values ----
+        uint32_t gen = 0;
+        // per child slot: lookups kr/kf, the index of the second lookup, the
+        // first and second values read, the direction
+        int ckr[4], ckf[4], cv1[4], cv2[4], cdir[4];
+        uint32_t cidx2[4];
+        int ng = 0;
+        bool regfirst = true; // read the regular lookup first (else the reflected)
+        if (st == ST_FRESH) {
+            const int br = s.br, bc = s.bc;
+            const uint32_t legal = (br > 0 ? 1u : 0u) | (bc < W - 1 ? 2u : 0u) | (bc > 0 ? 4u : 0u) |
+                                   (br < W - 1 ? 8u : 0u);
+            gen = legal & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
+            if constexpr (HEUR == 1) {
+                // lane-slot j reads the lane's j-th generated child, so the
+                // wave runs as many slots as its largest family (3 below the
+                // root), not one per direction.  Only one lookup per child
+                // is read here: that of the partition whose sum is larger at
+                // the node (before a pending move's values: a heuristic
+                // choice); the other is read only for children the first
+                // does not already cut off (the reference's early cut,
+                // src_horie/idas_pdb.cu:228-231, regular sum first).
+                ng = __builtin_popcount(gen);
+                regfirst = s.hs.A >= s.hs.B;
+                const int bl = br * W + bc;
+                const PdbGeo &G = *P.geo;
+                uint32_t rem = gen;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    ckr[j] = ckf[j] = cv1[j] = cv2[j] = cdir[j] = 0;
+                    cidx2[j] = 0;
+                    if (!__ballot(j < ng)) continue;
+                    if (j < ng) {
+                        const int d = __builtin_ctz(rem);
+                        rem &= rem - 1;
+                        const int from = bl + (d == DIR_UP ? -W : (d == DIR_DOWN ? W : (d == DIR_LEFT ? -1 : 1)));
+                        const int t = s.bd.tile(from);
+                        const uint2 wa = G.w[t][from], wb = G.w[t][bl];
+                        const int kr = (int)(wa.x >> 28), kf = (int)(wa.y >> 28);
+                        const uint32_t ir = sel4(s.hs.i0, s.hs.i1, s.hs.i2, s.hs.i3, kr) + (wb.x - wa.x);
+                        const uint32_t iff = sel4(s.hs.j0, s.hs.j1, s.hs.j2, s.hs.j3, kf) + (wb.y - wa.y);
+                        cdir[j] = d;
+                        ckr[j] = kr;
+                        ckf[j] = kf;
+                        cv1[j] = pdb_read(P, regfirst ? ir : iff);
+                        cidx2[j] = regfirst ? iff : ir;
+                    }
+                }
+            }
         }
-        // ---- expand: f-tests of the children ----
+        if constexpr (HEUR == 1) {
+            if (pmv) move_pdb_finish(s.hs, pm);
+        }
+
+        // ---- expand, second half: goal test, count, f-tests ----
+        bool expanded = false;
         if (st == ST_FRESH) {
             const int h = s.hs.h;
             bool xp = true;
@@ -382,10 +442,8 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                 }
             }
             if (xp) {
+                expanded = true;
                 const int br = s.br, bc = s.bc;
-                uint32_t legal = (br > 0 ? 1u : 0u) | (bc < W - 1 ? 2u : 0u) | (bc > 0 ? 4u : 0u) |
-                                 (br < W - 1 ? 8u : 0u);
-                uint32_t gen = legal & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
                 cnt += (uint32_t)__builtin_popcount(gen);
                 uint32_t pass = gen;
                 uint32_t cvals = 0, cvals2 = 0; // PDB values of children 0,1 / 2,3 (16 bits each)
@@ -410,30 +468,38 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                 } else {
                     // the PDB max is not consistent (lookups take the minimum
                     // over blank cells): h may jump by 3, so every generated
-                    // child gets its own f-test; the gathers of all children
-                    // are independent and issue together.  Lane-slot j tests
-                    // the lane's j-th generated child, so the wave runs as
-                    // many slots as its largest family (3 below the root),
-                    // not one per direction.
+                    // child gets its own f-test
                     pass = 0;
-                    const int ng = __builtin_popcount(gen);
-                    uint32_t rem = gen;
+                    const HState<1> &hs = s.hs;
+                    // the first partition's sum of each child; the second
+                    // lookup is read only where that sum fits
+                    uint32_t need = 0;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        if (!__ballot(j < ng)) break;
                         if (j < ng) {
-                            const int d = __builtin_ctz(rem);
-                            rem &= rem - 1;
-                            const int from = bl + (d == DIR_UP ? -W : (d == DIR_DOWN ? W : (d == DIR_LEFT ? -1 : 1)));
-                            const int t = s.bd.tile(from);
-                            uint32_t v;
-                            const int hc = pdb_child_h(P, s.hs, t, from, bl, v);
+                            const int s1 = regfirst ? hs.A - byte_of(hs.v0, ckr[j]) + cv1[j]
+                                                    : hs.B - byte_of(hs.v1, ckf[j]) + cv1[j];
+                            if (g + 1 + s1 <= L) {
+                                need |= 1u << j;
+                                cv2[j] = pdb_read(P, cidx2[j]);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if ((need >> j) & 1u) {
+                            const int d = cdir[j];
+                            const int vr = regfirst ? cv1[j] : cv2[j];
+                            const int vf = regfirst ? cv2[j] : cv1[j];
+                            const int A = hs.A - byte_of(hs.v0, ckr[j]) + vr;
+                            const int B = hs.B - byte_of(hs.v1, ckf[j]) + vf;
+                            const int hc = A > B ? A : B;
                             pass |= (g + 1 + hc <= L) ? (1u << d) : 0u;
-                            const uint32_t sh = 16u * (uint32_t)(d & 1);
+                            const uint32_t v = ((uint32_t)vr | ((uint32_t)vf << 8)) << (16u * (uint32_t)(d & 1));
                             if (d < 2)
-                                cvals |= v << sh;
+                                cvals |= v;
                             else
-                                cvals2 |= v << sh;
+                                cvals2 |= v;
                         }
                     }
                 }
@@ -462,6 +528,26 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                     st = ST_POP;
                 }
             }
+        }
+
+        // ---- items whose subtree is done (after the expansion, so that its
+        // fence does not wait on the step's table reads) ----
+        if (finish) {
+            if (cnt) {
+                atomicAdd(&p.cnt[inst], (unsigned long long)cnt);
+                atomicAdd(&p.load[(size_t)inst * p.capl + origin], (unsigned long long)cnt);
+            }
+            __threadfence();
+            unsigned old = atomicSub(&p.pend[inst], 1u);
+            if (old == 1u) finish_iteration(p, inst);
+            cnt = 0;
+            st = ST_IDLE;
+        }
+        if (p.prof) {
+            pf_steps += 1;
+            pf_idle += (st == ST_IDLE || st == ST_DRAINED) ? 1u : 0u;
+            pf_exp += expanded ? 1u : 0u;
+            pf_pop += (!expanded && (st == ST_POP || st == ST_DESC)) ? 1u : 0u;
         }
     }
 
