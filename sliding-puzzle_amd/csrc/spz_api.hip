@@ -884,9 +884,9 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     ctx->stats[11] = (double)n;
     ctx->paths_n = paths ? n : 0;
     if (p.prof) {
-        unsigned long long pf[4];
+        unsigned long long pf[7];
         HIPCHK(hipMemcpy(pf, p.prof, sizeof pf, hipMemcpyDeviceToHost));
-        for (int k = 0; k < 4; ++k) ctx->stats[16 + k] = (double)pf[k];
+        for (int k = 0; k < 7; ++k) ctx->stats[16 + k] = (double)pf[k];
     }
     ctx->stats[12] = all_nodes;
     return SPZ_OK;

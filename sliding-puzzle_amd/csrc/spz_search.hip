@@ -25,6 +25,9 @@ namespace spz {
 // as its deepest popping lane needs, so more pops per step cost every lane
 // (measured on MI355X: 2 beats 1, 3, 4 and 6 for both heuristics)
 constexpr int kPops = 2;
+// sibling-value cache of the PDB search: per lane, the deepest kSibLevels
+// stack levels (ring, tagged with the level); see k_round
+constexpr int kSibLevels = 8;
 
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -184,6 +187,20 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     int sp = (int)(stw >> 8);
     // stack levels, then (width 5) the lanes' boards
     Pos<W, HEUR, typename HotBoard<W>::T> s;
+    // Sibling-value cache (PDB): when a lane enters the first passing child
+    // of a node, the values its f-test read for the next two passing
+    // children go into slot level % kSibLevels (tag = level in bits 24-31,
+    // 6+6 bits per sibling); moving into such a sibling later then reads no
+    // table.  An entry overwritten by a deeper level (or from an earlier
+    // round: LDS does not persist) fails its tag and the move reads the
+    // tables as before.
+    uint32_t *sibc = nullptr;
+    if constexpr (HEUR == 1) {
+        sibc = (uint32_t *)(lds + (size_t)MAXD * sstride * sizeof(ST) + (size_t)blockDim.x * kLdsBoardBytes) +
+               threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < kSibLevels; ++k) sibc[k * blockDim.x] = 0xffffffffu;
+    }
     if constexpr (W == 5) {
         s.bd.p = lds + (size_t)MAXD * sstride * sizeof(ST) + threadIdx.x * kLdsBoardBytes;
     } else {
@@ -277,6 +294,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     };
 
     uint32_t pf_idle = 0, pf_exp = 0, pf_pop = 0, pf_steps = 0;
+    uint32_t pf_sib = 0, pf_rd1 = 0, pf_rd2 = 0; // table reads: sibling moves, first, second partition
     // the restored context is complete before the loop, so the compiler's
     // wait tracking does not carry its loads into every iteration (where it
     // would wait on a move's table reads before issuing the f-test's)
@@ -333,6 +351,15 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                 R = Re & (Re - 1);
                 slot = sp - 1;
                 st = ST_FRESH;
+                if constexpr (HEUR == 1) {
+                    uint32_t &c = sibc[(slot % kSibLevels) * blockDim.x];
+                    const uint32_t cv = c;
+                    if ((cv >> 24) == (uint32_t)slot) {
+                        known = true;
+                        kv = (cv & 63u) | (((cv >> 6) & 63u) << 8);
+                        c = (cv & 0xff000000u) | ((cv >> 12) & 0xfffu); // the sibling after it moves up
+                    }
+                }
             } else {
                 --sp;
             }
@@ -344,6 +371,19 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
             slot = sp;
             known = true;
             kv = (((down >= 2) ? dval2 : dval) >> (16 * (down & 1))) & 0xffffu;
+            if constexpr (HEUR == 1) {
+                if (R) {
+                    // values of the next two passing children, 6 + 6 bits each
+                    const int s1 = __builtin_ctz(R);
+                    const uint32_t R2 = R & (R - 1);
+                    const int s2 = R2 ? __builtin_ctz(R2) : s1;
+                    const uint32_t v1 = (((s1 >= 2) ? dval2 : dval) >> (16 * (s1 & 1))) & 0xffffu;
+                    const uint32_t v2 = (((s2 >= 2) ? dval2 : dval) >> (16 * (s2 & 1))) & 0xffffu;
+                    const uint32_t p1 = (v1 & 63u) | (((v1 >> 8) & 63u) << 6);
+                    const uint32_t p2 = (v2 & 63u) | (((v2 >> 8) & 63u) << 6);
+                    sibc[(slot % kSibLevels) * blockDim.x] = ((uint32_t)slot << 24) | p1 | (p2 << 12);
+                }
+            }
         }
         // ---- one move down per lane and step, both cases on one path.  For
         // the PDB the move's values are applied after the f-test's table
@@ -353,6 +393,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
         if (down >= 0) {
             if constexpr (HEUR == 1) {
                 uint32_t ov;
+                if (p.prof) pf_sib += known ? 0u : 1u;
                 pm = move_pdb_begin<W>(s, down, P, known, kv, ov);
                 pmv = true;
                 stk[slot * sstride] = SE::make(down, R, ov);
@@ -482,6 +523,10 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                             if (g + 1 + s1 <= L) {
                                 need |= 1u << j;
                                 cv2[j] = pdb_read(P, cidx2[j]);
+                            }
+                            if (p.prof) {
+                                pf_rd1 += 1u;
+                                pf_rd2 += (g + 1 + s1 <= L) ? 1u : 0u;
                             }
                         }
                     }
@@ -638,12 +683,18 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
             pf_idle += __shfl_xor(pf_idle, off);
             pf_exp += __shfl_xor(pf_exp, off);
             pf_pop += __shfl_xor(pf_pop, off);
+            pf_sib += __shfl_xor(pf_sib, off);
+            pf_rd1 += __shfl_xor(pf_rd1, off);
+            pf_rd2 += __shfl_xor(pf_rd2, off);
         }
         if (lane == 0) {
             atomicAdd(&p.prof[0], (unsigned long long)pf_idle);
             atomicAdd(&p.prof[1], (unsigned long long)pf_exp);
             atomicAdd(&p.prof[2], (unsigned long long)pf_pop);
             atomicAdd(&p.prof[3], (unsigned long long)pf_steps);
+            atomicAdd(&p.prof[4], (unsigned long long)pf_sib);
+            atomicAdd(&p.prof[5], (unsigned long long)pf_rd1);
+            atomicAdd(&p.prof[6], (unsigned long long)pf_rd2);
         }
     }
     uint64_t busy_mask = __ballot(busy);
@@ -978,9 +1029,11 @@ hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo
     return launch_round_t<5, 1>(p, geo, wg, s);
 }
 
-// dynamic LDS of k_round: the stack levels, then (width 5) the boards
+// dynamic LDS of k_round: the stack levels, then (width 5) the boards, then
+// (PDB) the sibling-value cache
 size_t round_lds_bytes(int width, int maxd, int wg, int heur) {
-    return (size_t)maxd * wg * (heur == 1 ? 2 : 1) + (width == 5 ? (size_t)kLdsBoardBytes * wg : 0);
+    return (size_t)maxd * wg * (heur == 1 ? 2 : 1) + (width == 5 ? (size_t)kLdsBoardBytes * wg : 0) +
+           (heur == 1 ? (size_t)kSibLevels * 4 * wg : 0);
 }
 
 const void *round_kernel_ptr(int width, int heur) {

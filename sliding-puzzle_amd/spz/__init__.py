@@ -405,4 +405,5 @@ class Solver:
                 "overflow_items": s[7], "round_kernel_ms": s[8], "round_launches": s[9],
                 "split_ms": s[10], "instances": s[11], "nodes": s[12],
                 "setup_ms": s[13], "rounds_wall_ms": s[14], "graph": s[15],
-                "prof_idle": s[16], "prof_expand": s[17], "prof_pop": s[18], "prof_wave_steps": s[19]}
+                "prof_idle": s[16], "prof_expand": s[17], "prof_pop": s[18], "prof_wave_steps": s[19],
+                "prof_sibling_moves": s[20], "prof_reads_first": s[21], "prof_reads_second": s[22]}
