@@ -22,8 +22,9 @@
 namespace spz {
 
 // stack levels a lane may pop in one step: a wave runs the pop loop as often
-// as its deepest popping lane needs, so more pops per step cost every lane
-// (measured on MI355X: 2 beats 1, 3, 4 and 6 for both heuristics)
+// as its deepest popping lane needs, and each pop is a chain of dependent LDS
+// reads, so more pops per step lengthen every lane's step (measured on
+// MI355X: 2 beats 1, 3, 4, 6, 8 and 12 for both heuristics)
 constexpr int kPops = 2;
 // sibling-value cache of the PDB search: per lane, the deepest kSibLevels
 // stack levels (ring, tagged with the level); see k_round
