@@ -206,7 +206,9 @@ int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens);
  *  16-19 with env SPZ_PROFILE=1: lane-steps idle, expanding, only popping,
  *        and wave-steps (x64 = lane-steps) of the search rounds;
  *  20-22 with SPZ_PROFILE=1 (PDB): moves into a sibling (2 table reads each),
- *        children's first-partition reads, second-partition reads */
+ *        children's first-partition reads, second-partition reads
+ * With env SPZ_TRACE=1 each poll of the round loop prints elapsed ms, busy
+ * lanes, active instances and queued items to stderr. */
 int spz_last_stats(spz_ctx *ctx, double *stats);
 
 const char *spz_last_error(spz_ctx *ctx);

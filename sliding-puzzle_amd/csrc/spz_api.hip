@@ -758,6 +758,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     // work vanishes while instances are still active (a bookkeeping bug).
     const int per_poll = 4;
     const double limit_s = ctx->timeout_s;
+    const bool trace = getenv("SPZ_TRACE") && atoi(getenv("SPZ_TRACE")) > 0; // per-poll progress on stderr
     int stalled = 0;
     const bool timing = ctx->cfg.time_kernels != 0;
     hipEvent_t ev[2 * per_poll];
@@ -824,6 +825,10 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
             rc = fail(ctx, SPZ_E_HIP, "search rounds: %s", hipGetErrorString(e));
             break;
         }
+        if (trace)
+            fprintf(stderr, "spz poll %ld: %.1f ms, busy lanes %u, active instances %u, queued %llu\n", rounds,
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_rounds).count(),
+                    h_ctl[12], h_ctl[1], (unsigned long long)(h_qc[2] - h_qc[0]));
         if (h_ctl[3]) {
             rc = fail(ctx, SPZ_E_CAPACITY, "work queue overflow: raise queue_items (now %zu items)", ctx->qcap);
             break;
