@@ -351,8 +351,10 @@ def main():
             "what": "uniform random u8 gathers over the two 244 MB direct tables (spz_gather_probe)"}
 
     if not args.no_plain and not per_instance:
-        # plain GPU IDA*: one instance per launch, uniform static split
-        plain = spz.Solver(width=width, heuristic=heur, frontier_target=FRONTIER[args.config])
+        # plain GPU IDA*: one instance per launch, uniform static split, one
+        # iteration at a time (the reference's plain method)
+        plain = spz.Solver(width=width, heuristic=heur, frontier_target=FRONTIER[args.config],
+                           lookahead=False)
         if heur == spz.H_PDB666:
             plain.build_pdb()
         t = time.perf_counter()

@@ -88,6 +88,15 @@ typedef struct spz_config {
     int64_t queue_items;     /* work-queue capacity in items (0 = auto) */
     int32_t record_paths;    /* 1: record one optimal solution path per instance
                                 (spz_get_paths); needs max_split_depth <= 32 */
+    int32_t lookahead;       /* 1 (default): while lanes are idle, start an
+                                instance's next iteration beside the one that
+                                is draining; counts and lengths are unchanged
+                                (each iteration counts apart, results resolve
+                                in order), work of a next iteration that turns
+                                out not to be needed is dropped.  Off under
+                                SPZ_LB_PREVLOAD and sample logging.  0: one
+                                iteration at a time, as in the reference
+                                (src/24puzzle.cc:146-152) */
 } spz_config;
 
 typedef struct spz_result {
@@ -207,6 +216,7 @@ int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens);
  *        and wave-steps (x64 = lane-steps) of the search rounds;
  *  20-22 with SPZ_PROFILE=1 (PDB): moves into a sibling (2 table reads each),
  *        children's first-partition reads, second-partition reads
+ *  23 iterations started as lookahead
  * With env SPZ_TRACE=1 each poll of the round loop prints elapsed ms, busy
  * lanes, active instances and queued items to stderr. */
 int spz_last_stats(spz_ctx *ctx, double *stats);

@@ -18,6 +18,7 @@ hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo
 size_t round_lds_bytes(int width, int maxd, int wg, int heur);
 const void *round_kernel_ptr(int width, int heur);
 hipError_t launch_restart(const KParams &p, hipStream_t s);
+hipError_t launch_spec(const KParams &p, hipStream_t s);
 hipError_t launch_control(const KParams &p, hipStream_t s);
 hipError_t launch_split_level(int width, int heur, const KParams &p, const PdbGeo *geo, const Item *lvl_in,
                               const unsigned *n_in, Item *lvl_out, unsigned *n_out, unsigned *nfront,
@@ -86,12 +87,14 @@ struct spz_ctx {
     size_t qcap = 0;
     // per instance
     DBuf roots, L, it, cnt, pend, found, done, opt, iter_counts, restart_list;
+    DBuf itn, drained, nlive, Lbase, listed, want_next; // iteration slots (lookahead)
     DBuf frontier, nfront, topgen, top_goal, frozen, lvl[2], nlvl[2];
     // load balancing
     DBuf c_origin, load, samples, nsamp, pred;
     // solution paths (record_paths)
     DBuf qpath, c_ipath, fpath, lpath[2], sol, sol_len;
-    int paths_n = 0; // instances of the last solve with recorded paths
+    int paths_n = 0;              // instances of the last solve with recorded paths
+    std::vector<int> paths_slot; // their final iteration's slot
     MlpState mlp;
     bool mlp_alloced = false;
 };
@@ -165,6 +168,7 @@ void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic) {
     cfg->log_samples = 0;
     cfg->use_graph = 1;
     cfg->queue_items = 0;
+    cfg->lookahead = 1;
 }
 
 // Steps per lane and round when the configuration leaves it open: shorter
@@ -386,7 +390,9 @@ void spz_destroy(spz_ctx *ctx) {
                    &ctx->lvl[0],      &ctx->lvl[1],         &ctx->nlvl[0],        &ctx->nlvl[1],
                    &ctx->c_origin,    &ctx->load,           &ctx->samples,        &ctx->nsamp,
                    &ctx->pred,        &ctx->qpath,          &ctx->c_ipath,        &ctx->fpath,
-                   &ctx->lpath[0],    &ctx->lpath[1],       &ctx->sol,            &ctx->sol_len};
+                   &ctx->lpath[0],    &ctx->lpath[1],       &ctx->sol,            &ctx->sol_len,
+                   &ctx->itn,         &ctx->drained,        &ctx->nlive,          &ctx->Lbase,
+                   &ctx->listed,      &ctx->want_next};
     for (DBuf *b : all) b->release();
     if (ctx->mlp_alloced) mlp_free(ctx->mlp);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -513,6 +519,12 @@ static KParams base_params(spz_ctx *ctx, int n, int max_iters, int capl) {
     p.roots = ctx->roots.as<Item>();
     p.L = ctx->L.as<int>();
     p.it = ctx->it.as<int>();
+    p.itn = ctx->itn.as<int>();
+    p.drained = ctx->drained.as<int>();
+    p.nlive = ctx->nlive.as<int>();
+    p.Lbase = ctx->Lbase.as<int>();
+    p.listed = ctx->listed.as<int>();
+    p.want_next = ctx->want_next.as<int>();
     p.cnt = ctx->cnt.as<unsigned long long>();
     p.pend = ctx->pend.as<unsigned>();
     p.found = ctx->found.as<int>();
@@ -617,11 +629,17 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
 
     HIPCHK(ctx->q.ensure(qcap * sizeof(Item)));
     HIPCHK(ctx->roots.ensure(N * sizeof(Item)));
-    HIPCHK(ctx->L.ensure(N * 4));
+    HIPCHK(ctx->L.ensure(2 * N * 4));
+    HIPCHK(ctx->itn.ensure(2 * N * 4));
+    HIPCHK(ctx->cnt.ensure(2 * N * 8));
+    HIPCHK(ctx->pend.ensure(2 * N * 4));
+    HIPCHK(ctx->found.ensure(2 * N * 4));
+    HIPCHK(ctx->drained.ensure(2 * N * 4));
     HIPCHK(ctx->it.ensure(N * 4));
-    HIPCHK(ctx->cnt.ensure(N * 8));
-    HIPCHK(ctx->pend.ensure(N * 4));
-    HIPCHK(ctx->found.ensure(N * 4));
+    HIPCHK(ctx->nlive.ensure(N * 4));
+    HIPCHK(ctx->Lbase.ensure(N * 4));
+    HIPCHK(ctx->listed.ensure(N * 4));
+    HIPCHK(ctx->want_next.ensure(N * 4));
     HIPCHK(ctx->done.ensure(N * 4));
     HIPCHK(ctx->opt.ensure(N * 4));
     HIPCHK(ctx->iter_counts.ensure(N * (size_t)max_iters * 8));
@@ -646,6 +664,9 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
 
     KParams p = base_params(ctx, n, max_iters, capl);
     p.n_lanes = (int)lanes;
+    // lookahead runs two iterations of an instance at once; the per-node
+    // loads (load-based LB, training samples) describe one iteration at a time
+    p.lookahead = ctx->cfg.lookahead != 0 && ctx->cfg.lb_policy != SPZ_LB_PREVLOAD && p.max_samples == 0 ? 1 : 0;
 
     // solution paths: every queued item carries its moves from the root
     const bool paths = ctx->cfg.record_paths != 0;
@@ -657,11 +678,11 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         HIPCHK(ctx->c_ipath.ensure((size_t)kPathWords * lanes * 4));
         HIPCHK(ctx->fpath.ensure(front_items * 8));
         for (int k = 0; k < 2; ++k) HIPCHK(ctx->lpath[k].ensure(front_items * 8));
-        HIPCHK(ctx->sol.ensure(N * kPathWords * 4));
-        HIPCHK(ctx->sol_len.ensure(N * 4));
+        HIPCHK(ctx->sol.ensure(2 * N * kPathWords * 4)); // per iteration slot
+        HIPCHK(ctx->sol_len.ensure(2 * N * 4));
         HIPCHK(hipMemsetAsync(ctx->lpath[0].p, 0, front_items * 8, s)); // the roots' empty paths
-        HIPCHK(hipMemsetAsync(ctx->sol.p, 0, N * kPathWords * 4, s));
-        HIPCHK(hipMemsetAsync(ctx->sol_len.p, 0xff, N * 4, s)); // -1: none yet
+        HIPCHK(hipMemsetAsync(ctx->sol.p, 0, 2 * N * kPathWords * 4, s));
+        HIPCHK(hipMemsetAsync(ctx->sol_len.p, 0xff, 2 * N * 4, s)); // -1: none yet
         p.qpath = ctx->qpath.as<uint32_t>();
         p.c_ipath = ctx->c_ipath.as<uint32_t>();
         p.fpath = ctx->fpath.as<unsigned long long>();
@@ -718,18 +739,25 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     HIPCHK(hipStreamSynchronize(s));
     ctx->stats[10] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_split).count();
 
-    // per-instance iteration state
+    // per-instance iteration state: every instance on the restart list with
+    // no iteration started (k_restart starts iteration 0)
     {
-        std::vector<int> Lh(N), zero(N, 0), nog(N, kNoGoal), neg(N, -1), list(N);
+        std::vector<int> Lh(N), nog(2 * N, kNoGoal), neg(N, -1), list(N), ones(N, 1);
         for (int i = 0; i < n; ++i) {
             Lh[i] = meta_h(roots[i].meta);
             list[i] = i;
         }
-        HIPCHK(hipMemcpyAsync(ctx->L.p, Lh.data(), N * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(ctx->it.p, zero.data(), N * 4, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemsetAsync(ctx->cnt.p, 0, N * 8, s));
-        HIPCHK(hipMemsetAsync(ctx->pend.p, 0, N * 4, s));
-        HIPCHK(hipMemcpyAsync(ctx->found.p, nog.data(), N * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(ctx->Lbase.p, Lh.data(), N * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(ctx->L.p, 0, 2 * N * 4, s));
+        HIPCHK(hipMemsetAsync(ctx->itn.p, 0, 2 * N * 4, s));
+        HIPCHK(hipMemsetAsync(ctx->cnt.p, 0, 2 * N * 8, s));
+        HIPCHK(hipMemsetAsync(ctx->pend.p, 0, 2 * N * 4, s));
+        HIPCHK(hipMemsetAsync(ctx->drained.p, 0, 2 * N * 4, s));
+        HIPCHK(hipMemcpyAsync(ctx->found.p, nog.data(), 2 * N * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(ctx->it.p, 0, N * 4, s));
+        HIPCHK(hipMemsetAsync(ctx->nlive.p, 0, N * 4, s));
+        HIPCHK(hipMemcpyAsync(ctx->listed.p, ones.data(), N * 4, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemsetAsync(ctx->want_next.p, 0, N * 4, s));
         HIPCHK(hipMemsetAsync(ctx->done.p, 0, N * 4, s));
         HIPCHK(hipMemcpyAsync(ctx->opt.p, neg.data(), N * 4, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemsetAsync(ctx->iter_counts.p, 0, N * (size_t)max_iters * 8, s));
@@ -779,6 +807,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
             if (timing) e = hipEventRecord(ev[2 * k], s);
             if (e == hipSuccess) e = launch_round(w, heur, p, ctx->geo_d.as<PdbGeo>(), ctx->wg, s);
             if (timing && e == hipSuccess) e = hipEventRecord(ev[2 * k + 1], s);
+            if (e == hipSuccess && p.lookahead) e = launch_spec(p, s);
             if (e == hipSuccess) e = launch_restart(p, s);
             if (e == hipSuccess) e = launch_control(p, s);
         }
@@ -825,10 +854,15 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
             rc = fail(ctx, SPZ_E_HIP, "search rounds: %s", hipGetErrorString(e));
             break;
         }
-        if (trace)
-            fprintf(stderr, "spz poll %ld: %.1f ms, busy lanes %u, active instances %u, queued %llu\n", rounds,
+        if (trace) {
+            fprintf(stderr, "spz poll %ld: %.1f ms, busy lanes %u, active instances %u, queued %llu", rounds,
                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_rounds).count(),
                     h_ctl[12], h_ctl[1], (unsigned long long)(h_qc[2] - h_qc[0]));
+            unsigned long long pf[4];
+            if (p.prof && hipMemcpy(pf, p.prof, sizeof pf, hipMemcpyDeviceToHost) == hipSuccess)
+                fprintf(stderr, ", lane-steps idle %llu expanding %llu popping %llu", pf[0], pf[1], pf[2]);
+            fputc('\n', stderr);
+        }
         if (h_ctl[3]) {
             rc = fail(ctx, SPZ_E_CAPACITY, "work queue overflow: raise queue_items (now %zu items)", ctx->qcap);
             break;
@@ -863,6 +897,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     ctx->stats[2] = (double)h_ctl[7];
     ctx->stats[3] = (double)lanes;
     ctx->stats[7] = (double)h_ctl[8];
+    ctx->stats[23] = (double)h_ctl[13];
     hipHostFree(h_ctl);
 
     std::vector<int> opt(N), its(N);
@@ -888,6 +923,10 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     }
     ctx->stats[11] = (double)n;
     ctx->paths_n = paths ? n : 0;
+    if (paths) {
+        ctx->paths_slot.resize(N);
+        for (int i = 0; i < n; ++i) ctx->paths_slot[i] = its[i] & 1;
+    }
     if (p.prof) {
         unsigned long long pf[7];
         HIPCHK(hipMemcpy(pf, p.prof, sizeof pf, hipMemcpyDeviceToHost));
@@ -902,14 +941,17 @@ int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens) 
     if (ctx->paths_n <= 0) return fail(ctx, SPZ_E_ARG, "the last solve recorded no paths (config record_paths)");
     HIPCHK(hipSetDevice(ctx->cfg.device));
     const size_t n = (size_t)ctx->paths_n;
-    std::vector<int> len(n);
-    std::vector<uint32_t> w(n * kPathWords);
-    HIPCHK(hipMemcpy(len.data(), ctx->sol_len.p, n * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(w.data(), ctx->sol.p, n * kPathWords * 4, hipMemcpyDeviceToHost));
+    // one path per iteration slot; the instance's answer is in the slot of
+    // its last iteration
+    std::vector<int> len(2 * n);
+    std::vector<uint32_t> w(2 * n * kPathWords);
+    HIPCHK(hipMemcpy(len.data(), ctx->sol_len.p, 2 * n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(w.data(), ctx->sol.p, 2 * n * kPathWords * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; ++i) {
-        lens[i] = len[i];
+        const size_t sl = 2 * i + (size_t)ctx->paths_slot[i];
+        lens[i] = len[sl];
         for (int j = 0; j < max_len; ++j)
-            moves[i * max_len + j] = j < len[i] ? (uint8_t)((w[i * kPathWords + j / 16] >> (2 * (j % 16))) & 3u) : 0;
+            moves[i * max_len + j] = j < len[sl] ? (uint8_t)((w[sl * kPathWords + j / 16] >> (2 * (j % 16))) & 3u) : 0;
     }
     return SPZ_OK;
 }

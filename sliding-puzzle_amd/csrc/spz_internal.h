@@ -65,11 +65,22 @@ struct KParams {
     int max_iters;
     int mode;
     const Item *roots;           // [n_inst]
-    int *L;                      // current threshold
-    int *it;                     // current iteration index
-    unsigned long long *cnt;     // count of the current iteration
-    unsigned int *pend;          // items of the current iteration not finished
-    int *found;                  // min goal depth (kNoGoal if none)
+    // An instance runs its current iteration and, with lookahead, the next
+    // one beside it.  Iteration j lives in slot j & 1; the arrays marked
+    // [2 n_inst] are indexed instance * 2 + slot ("slot index"), which is
+    // also what items and lanes carry as their instance.
+    int *L;                      // [2 n_inst] threshold of the slot's iteration
+    int *itn;                    // [2 n_inst] iteration index held by the slot
+    unsigned long long *cnt;     // [2 n_inst] count of the slot's iteration
+    unsigned int *pend;          // [2 n_inst] its items not finished
+    int *found;                  // [2 n_inst] min goal depth (kNoGoal if none)
+    int *drained;                // [2 n_inst] 1 once pend reached zero
+    int *it;                     // [n_inst] current (oldest unresolved) iteration
+    int *nlive;                  // [n_inst] iterations started, not resolved (0-2)
+    const int *Lbase;            // [n_inst] threshold of iteration 0 (h of the root)
+    int *listed;                 // [n_inst] 1 while on the restart list
+    int *want_next;              // [n_inst] lookahead requested for the next restart
+    int lookahead;               // 1: start iteration it+1 while `it` drains
     int *done;
     int *opt;
     unsigned long long *iter_counts; // [n_inst][max_iters]
@@ -77,7 +88,8 @@ struct KParams {
     unsigned int *ctl;           // 0 n_restart, 1 active, 2 busy, 3 queue overflow,
                                  // 4 donate_per_lane, 5 budget, 6 rounds,
                                  // 7 donated, 8 overflow exports, 9/10 big/small
-                                 // budget, 12 busy at last round end
+                                 // budget, 12 busy at last round end,
+                                 // 13 lookahead iterations started
 
     // top tree of the root split (per instance)
     const Item *frontier;        // [n_inst][capl]
@@ -119,8 +131,8 @@ struct KParams {
     uint32_t *qpath;                 // [qcap][kPathWords] paths of the queued items
     uint32_t *c_ipath;               // [kPathWords][n_lanes] path of each lane's item root
     const unsigned long long *fpath; // [n_inst][capl] paths of the frontier nodes
-    uint32_t *sol;                   // [n_inst][kPathWords] one optimal path per instance
-    int *sol_len;                    // [n_inst] its length, -1 until recorded
+    uint32_t *sol;                   // [2 n_inst][kPathWords] one goal path per slot
+    int *sol_len;                    // [2 n_inst] its length, -1 until recorded
 };
 
 } // namespace spz

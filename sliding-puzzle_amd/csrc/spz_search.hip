@@ -44,28 +44,26 @@ __device__ __forceinline__ unsigned long long shfl_u64(unsigned long long v, int
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// Iteration bookkeeping when the last item of an instance's iteration ends.
-// Called by exactly one lane (the one whose pend decrement reached zero).
-__device__ void finish_iteration(const KParams &p, int inst) {
-    int it = p.it[inst];
-    unsigned long long c = atomicAdd(&p.cnt[inst], 0ull);
-    p.iter_counts[(size_t)inst * p.max_iters + it] = c;
-    int fd = atomicAdd(&p.found[inst], 0);
-    if (fd != kNoGoal) {
-        p.opt[inst] = fd;
-        p.done[inst] = 1;
-        atomicSub(&p.ctl[1], 1u);
-    } else if (it + 1 >= p.max_iters) {
-        p.opt[inst] = -1;
-        p.done[inst] = 1;
-        atomicSub(&p.ctl[1], 1u);
-    } else {
-        p.L[inst] += 2;
-        p.it[inst] = it + 1;
-        atomicExch(&p.cnt[inst], 0ull);
-        unsigned k = atomicAdd(&p.ctl[0], 1u);
+// The last item of a slot's iteration has ended.  Called by exactly one lane
+// (the one whose pend decrement reached zero): it marks the slot and puts the
+// instance on the restart list; k_restart resolves the instance's iterations
+// in order between rounds.
+__device__ void iteration_drained(const KParams &p, int isl) {
+    p.drained[isl] = 1;
+    const int inst = isl >> 1;
+    if (atomicExch(&p.listed[inst], 1) == 0) {
+        const unsigned k = atomicAdd(&p.ctl[0], 1u);
         p.restart_list[k] = inst;
     }
+}
+
+// A lookahead iteration whose work is no longer needed: the instance is
+// done, or its older iteration (the other slot, lower index) has found a
+// goal and so is the last one.  Lanes drop such items unsearched.
+__device__ __forceinline__ bool superseded(const KParams &p, int isl) {
+    if (p.done[isl >> 1]) return true;
+    const int o = isl ^ 1;
+    return p.found[o] != kNoGoal && p.itn[o] < p.itn[isl];
 }
 
 // Append one item.  pos is a reserved queue position.
@@ -211,6 +209,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     s.hs.h = 0;
     uint32_t root[4] = {0, 0, 0, 0};
     uint32_t rmeta = 0;
+    // inst is the item's slot index (instance * 2 + iteration slot)
     int g = 0, prev = DIR_NONE, inst = 0, L = 0;
     uint32_t cnt = 0, origin = 0;
 
@@ -248,6 +247,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
             s.hs.B = (int)((v1 & 0xff) + ((v1 >> 8) & 0xff) + ((v1 >> 16) & 0xff) + (v1 >> 24));
         }
         L = p.L[inst];
+        if (p.lookahead && superseded(p, inst)) sp = 0, st = ST_POP; // unwinds and finishes next step
         for (int k = 0; k < sp; ++k) stk[k * sstride] = cstk[(size_t)k * NL + gl];
     } else {
         st = ST_IDLE;
@@ -267,6 +267,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
         cnt = 0;
         st = ST_FRESH;
         if (mode == 1 && p.found[inst] != kNoGoal) st = ST_POP; // first-solution: abandon
+        if (p.lookahead && superseded(p, inst)) st = ST_POP;
     };
 
     // move down into child d (stack level sl, R = passing children still to
@@ -469,9 +470,9 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
             bool xp = true;
             if (h == 0) {
                 atomicMin(&p.found[inst], g);
-                // the first goal of the instance records its path (every goal
-                // found has the optimal depth: the first iteration with a goal
-                // has threshold = optimum)
+                // the first goal of the slot's iteration records its path
+                // (the first iteration with a goal has threshold = optimum,
+                // and every goal it finds has that depth)
                 if (p.sol && g <= kPathMax && atomicCAS(&p.sol_len[inst], -1, g) == -1) {
                     const uint32_t so = (uint32_t)inst * kPathWords;
                     path_copy(p.c_ipath, (uint32_t)gl, (uint32_t)NL, p.sol, so, 1);
@@ -581,11 +582,11 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
         if (finish) {
             if (cnt) {
                 atomicAdd(&p.cnt[inst], (unsigned long long)cnt);
-                atomicAdd(&p.load[(size_t)inst * p.capl + origin], (unsigned long long)cnt);
+                atomicAdd(&p.load[(size_t)(inst >> 1) * p.capl + origin], (unsigned long long)cnt);
             }
             __threadfence();
             unsigned old = atomicSub(&p.pend[inst], 1u);
-            if (old == 1u) finish_iteration(p, inst);
+            if (old == 1u) iteration_drained(p, inst);
             cnt = 0;
             st = ST_IDLE;
         }
@@ -675,7 +676,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     // keep the per-lane u32 count far from wrapping on long-lived items
     if (busy && cnt >= (1u << 30)) {
         atomicAdd(&p.cnt[inst], (unsigned long long)cnt);
-        atomicAdd(&p.load[(size_t)inst * p.capl + origin], (unsigned long long)cnt);
+        atomicAdd(&p.load[(size_t)(inst >> 1) * p.capl + origin], (unsigned long long)cnt);
         cnt = 0;
     }
     if (p.prof) {
@@ -738,19 +739,27 @@ __device__ __forceinline__ int path_f(const Item &it) {
 }
 
 // ---------------------------------------------------------------------------
-// k_restart: start the current iteration of every instance on the restart
-// list (one block each).  It
+// k_restart: between rounds, one block per instance on the restart list (a
+// slot of it drained, it is new, or lookahead was asked for).  Thread 0
+// resolves the instance's iterations in order: while the current iteration
+// has drained, its count is recorded and it ends the instance (goal found,
+// or iteration limit) or the next one becomes current -- already running if
+// it was started as lookahead.  An iteration is started when the instance has
+// none running, or as lookahead beside the current one when k_spec asked for
+// it.  Starting iteration j (threshold Lbase + 2j) in slot j & 1:
 //   1. logs (frontier node, previous threshold, nodes generated below it)
-//      samples of the iteration that just ended (learned-LB training data),
-//   2. adds the top-tree count of the new threshold from the f-histogram and
-//      notes goals in the top tree,
+//      samples of the iteration that ended (learned-LB training data; only
+//      without lookahead, where one iteration runs at a time),
+//   2. sets the slot's count to the top-tree count of the threshold from the
+//      f-histogram and notes a goal in the top tree,
 //   3. queues the frontier nodes whose path-f fits, ordered by a key in
 //      decreasing order (counting sort over 64 half-log2 buckets): frontier
 //      order (uniform), last iteration's load (the reference's load-based
 //      static LB, src/15puzzle_block_parallel.cu:490-563), or the learned
 //      prediction -- longest expected subtree first,
 //   4. clears the per-frontier-node loads.
-// An iteration with no frontier work completes here.
+// An iteration with no frontier work is drained at once.  Lanes run no
+// kernel meanwhile, so the resolution needs no atomics on instance state.
 __device__ __forceinline__ int lb_bucket(const KParams &p, int inst, unsigned i, int it) {
     float key = 0.f;
     if (p.lb == 2) {
@@ -762,12 +771,44 @@ __device__ __forceinline__ int lb_bucket(const KParams &p, int inst, unsigned i,
     return b < 0 ? 0 : (b > 63 ? 63 : b);
 }
 
+// thread 0 of k_restart: resolve drained iterations; returns the index of an
+// iteration to start, or -1
+__device__ int resolve_instance(const KParams &p, int inst) {
+    for (;;) {
+        if (p.done[inst]) return -1;
+        const int cur = p.it[inst];
+        const int nl = p.nlive[inst];
+        if (nl == 0) {
+            p.nlive[inst] = 1;
+            return cur;
+        }
+        const int isc = inst * 2 + (cur & 1);
+        if (!p.drained[isc]) {
+            if (p.want_next[inst] && nl == 1 && cur + 1 < p.max_iters && p.found[isc] == kNoGoal) {
+                p.want_next[inst] = 0;
+                p.nlive[inst] = 2;
+                atomicAdd(&p.ctl[13], 1u);
+                return cur + 1;
+            }
+            return -1;
+        }
+        p.iter_counts[(size_t)inst * p.max_iters + cur] = p.cnt[isc];
+        if (p.found[isc] != kNoGoal || cur + 1 >= p.max_iters) {
+            p.opt[inst] = p.found[isc] != kNoGoal ? p.found[isc] : -1;
+            p.done[inst] = 1;
+            atomicSub(&p.ctl[1], 1u);
+            return -1;
+        }
+        p.it[inst] = cur + 1;
+        p.nlive[inst] = nl - 1;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_restart(KParams p) {
     __shared__ unsigned long long s_cnt;
-    __shared__ unsigned s_n;
     __shared__ unsigned s_bucket[64];
     __shared__ unsigned long long s_base;
-    __shared__ int s_more;
+    __shared__ int s_j;
     const unsigned n_list = p.ctl[0];
     if (blockIdx.x >= n_list) return;
     const int inst = p.restart_list[blockIdx.x];
@@ -776,10 +817,14 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
     unsigned long long *load = p.load + (size_t)inst * p.capl;
 
     for (;;) {
-        const int L = p.L[inst];
-        const int it = p.it[inst];
+        if (threadIdx.x == 0) s_j = resolve_instance(p, inst);
+        __syncthreads();
+        const int it = s_j;
+        if (it < 0) break;
+        const int isl = inst * 2 + (it & 1);
+        const int L = p.Lbase[inst] + 2 * it;
         // 1. samples of the iteration that ended
-        if (it > 0 && p.max_samples > 0) {
+        if (it > 0 && p.max_samples > 0 && !p.lookahead) {
             for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) {
                 if (path_f(F[i]) <= L - 2) {
                     unsigned k = atomicAdd(p.n_samples, 1u);
@@ -794,10 +839,7 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
                 }
             }
         }
-        if (threadIdx.x == 0) {
-            s_cnt = 0;
-            s_n = 0;
-        }
+        if (threadIdx.x == 0) s_cnt = 0;
         for (int b = threadIdx.x; b < 64; b += blockDim.x) s_bucket[b] = 0;
         __syncthreads();
         // 2. top tree: sum of gen over path-f <= L
@@ -816,17 +858,20 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
                 s_bucket[b] = off;
                 off += n;
             }
-            s_n = off;
             s_base = off ? q_reserve(p, (unsigned long long)off) : 0ull;
-            if (p.top_goal[inst] <= L) atomicMin(&p.found[inst], p.top_goal[inst]);
-            p.cnt[inst] = s_cnt;
-            p.pend[inst] = off;
+            p.L[isl] = L;
+            p.itn[isl] = it;
+            p.found[isl] = p.top_goal[inst] <= L ? p.top_goal[inst] : kNoGoal;
+            p.cnt[isl] = s_cnt;
+            p.pend[isl] = off;
+            p.drained[isl] = off == 0 ? 1 : 0;
         }
         __syncthreads();
         for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) {
             Item x = F[i];
             if (path_f(x) <= L) {
                 unsigned k = atomicAdd(&s_bucket[lb_bucket(p, inst, i, it)], 1u);
+                x.inst = (uint32_t)isl;
                 x.x1 = i;
                 p.q[(s_base + k) & p.qmask] = x;
                 if (p.qpath)
@@ -834,31 +879,50 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
                                   (uint32_t)((s_base + k) & p.qmask) * kPathWords);
             }
         }
-        __syncthreads();
         // 4. loads of the new iteration start from zero
         for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) load[i] = 0;
-        if (threadIdx.x == 0) {
-            s_more = 0;
-            if (s_n == 0) {
-                // nothing to search below the split: the iteration is complete
-                p.iter_counts[(size_t)inst * p.max_iters + it] = s_cnt;
-                if (p.found[inst] != kNoGoal) {
-                    p.opt[inst] = p.found[inst];
-                    p.done[inst] = 1;
-                    atomicSub(&p.ctl[1], 1u);
-                } else if (it + 1 >= p.max_iters) {
-                    p.opt[inst] = -1;
-                    p.done[inst] = 1;
-                    atomicSub(&p.ctl[1], 1u);
-                } else {
-                    p.L[inst] = L + 2;
-                    p.it[inst] = it + 1;
-                    s_more = 1;
-                }
-            }
-        }
         __syncthreads();
-        if (!s_more) break;
+    }
+    if (threadIdx.x == 0) {
+        p.want_next[inst] = 0;
+        p.listed[inst] = 0;
+    }
+}
+
+// k_spec (lookahead): after a round, while idle lanes outnumber the queued
+// items by more than 1/32 of the lanes, put instances that run a single
+// iteration (no goal yet) on the restart list with lookahead asked for, until
+// their frontiers would fill the idle lanes.  Instances are taken from a
+// start that rotates with the round number.  Only the schedule changes: each
+// iteration's count is its own, and the result is resolved in order.
+__global__ void __launch_bounds__(256) k_spec(KParams p) {
+    __shared__ unsigned long long s_room, s_used;
+    if (threadIdx.x == 0) {
+        const unsigned long long head = p.q_ctl[0], lim = p.q_ctl[1], tail = p.q_ctl[2];
+        const unsigned long long qlen = tail - (head < lim ? head : lim);
+        const unsigned long long lanes = (unsigned long long)p.n_lanes, busy = p.ctl[2];
+        const unsigned long long taken = busy + qlen;
+        s_room = (taken < lanes && lanes - taken >= lanes / 32) ? lanes - taken : 0ull;
+        s_used = 0;
+    }
+    __syncthreads();
+    if (s_room == 0) return;
+    const int n = p.n_inst;
+    const int start = (int)(p.ctl[6] % (unsigned)n);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        int inst = start + k;
+        if (inst >= n) inst -= n;
+        if (p.done[inst] || p.nlive[inst] != 1 || p.listed[inst]) continue;
+        const int cur = p.it[inst];
+        const int isc = inst * 2 + (cur & 1);
+        if (cur + 1 >= p.max_iters || p.found[isc] != kNoGoal || p.drained[isc]) continue;
+        const unsigned long long est = min(p.nfront[inst], (unsigned)p.capl);
+        if (atomicAdd(&s_used, est) >= s_room) continue;
+        if (atomicExch(&p.listed[inst], 1) == 0) {
+            p.want_next[inst] = 1;
+            const unsigned k2 = atomicAdd(&p.ctl[0], 1u);
+            p.restart_list[k2] = inst;
+        }
     }
 }
 
@@ -948,9 +1012,13 @@ __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *ge
     if ((int)it.x0 > f) f = (int)it.x0;
     if (s.hs.h == 0) {
         atomicMin(&top_goal[inst], f);
-        // levels run in depth order, so the first goal recorded is a shallowest one
-        if (p.sol && atomicCAS(&p.sol_len[inst], -1, g) == -1)
-            path_from_u64(lp_in[(size_t)inst * p.capl + j], p.sol, (uint32_t)inst * kPathWords);
+        // levels run in depth order, so the first goal recorded is a
+        // shallowest one; it goes to both iteration slots (the iteration that
+        // ends the instance, in either slot, finds no shallower goal)
+        if (p.sol)
+            for (int sl = inst * 2; sl < inst * 2 + 2; ++sl)
+                if (atomicCAS(&p.sol_len[sl], -1, g) == -1)
+                    path_from_u64(lp_in[(size_t)inst * p.capl + j], p.sol, (uint32_t)sl * kPathWords);
     }
     uint32_t legal = (br > 0 ? 1u : 0u) | (bc < W - 1 ? 2u : 0u) | (bc > 0 ? 4u : 0u) | (br < W - 1 ? 8u : 0u);
     uint32_t gen = legal & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
@@ -1045,6 +1113,11 @@ const void *round_kernel_ptr(int width, int heur) {
 
 hipError_t launch_restart(const KParams &p, hipStream_t s) {
     hipLaunchKernelGGL(k_restart, dim3(p.n_inst), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_spec(const KParams &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_spec, dim3(1), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

@@ -31,7 +31,8 @@ static void usage() {
                  "  --export-tab DIR       write pat24.1256712.tab and pat24.34891314.tab\n"
                  "  --lb uniform|prevload  dispatch order of frontier nodes\n"
                  "  --frontier N --wg N --lds-depth N --budget N --max-iters N\n"
-                 "  --autotune             tune the launch configuration on the files first\n");
+                 "  --autotune             tune the launch configuration on the files first\n"
+                 "  --no-lookahead         run each instance's iterations strictly one at a time\n");
 }
 
 static bool read_board(const std::string &path, std::vector<uint8_t> &b, int &width) {
@@ -52,7 +53,7 @@ int main(int argc, char **argv) {
     spz_config cfg;
     int heur = SPZ_H_MANHATTAN, mode = SPZ_MODE_ALL, lb = SPZ_LB_UNIFORM;
     int frontier = 2048, wg = 256, depth = 0, budget = 0, max_iters = 64;
-    bool batch = false, iters = false, autotune = false, path = false;
+    bool batch = false, iters = false, autotune = false, path = false, lookahead = true;
     std::string csv, cache, tabdir;
     std::vector<std::string> files;
     for (int i = 1; i < argc; ++i) {
@@ -94,6 +95,8 @@ int main(int argc, char **argv) {
             budget = std::atoi(next());
         } else if (a == "--max-iters") {
             max_iters = std::atoi(next());
+        } else if (a == "--no-lookahead") {
+            lookahead = false;
         } else if (a == "--autotune") {
             autotune = true;
         } else if (a == "-h" || a == "--help") {
@@ -126,6 +129,7 @@ int main(int argc, char **argv) {
     cfg.round_budget = budget;
     cfg.lb_policy = lb;
     cfg.record_paths = path ? 1 : 0;
+    cfg.lookahead = lookahead ? 1 : 0;
     spz_ctx *ctx = nullptr;
     int rc = spz_create(&ctx, &cfg);
     if (rc != SPZ_OK) {

@@ -47,7 +47,8 @@ class Config(ctypes.Structure):
                 ("lb_policy", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("time_kernels", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("log_samples", ctypes.c_int32),
-                ("queue_items", ctypes.c_int64), ("record_paths", ctypes.c_int32)]
+                ("queue_items", ctypes.c_int64), ("record_paths", ctypes.c_int32),
+                ("lookahead", ctypes.c_int32)]
 
 
 class _Result(ctypes.Structure):
@@ -184,7 +185,7 @@ class Solver:
                  workgroup_size: int = 256, lanes: int = 0, round_budget: int = 0,
                  lb_policy: int = LB_UNIFORM, device: int = 0, queue_items: int = 0,
                  time_kernels: bool = False, lds_stack_depth: int = 0, log_samples: int = 0,
-                 use_graph: bool = True, record_paths: bool = False):
+                 use_graph: bool = True, record_paths: bool = False, lookahead: bool = True):
         L = lib()
         cfg = Config()
         L.spz_default_config(ctypes.byref(cfg), width, heuristic)
@@ -202,6 +203,7 @@ class Solver:
         cfg.log_samples = log_samples
         cfg.use_graph = 1 if use_graph else 0
         cfg.record_paths = 1 if record_paths else 0
+        cfg.lookahead = 1 if lookahead else 0
         self.cfg = cfg
         self.width = width
         self._ctx = ctypes.c_void_p()
@@ -406,4 +408,5 @@ class Solver:
                 "split_ms": s[10], "instances": s[11], "nodes": s[12],
                 "setup_ms": s[13], "rounds_wall_ms": s[14], "graph": s[15],
                 "prof_idle": s[16], "prof_expand": s[17], "prof_pop": s[18], "prof_wave_steps": s[19],
-                "prof_sibling_moves": s[20], "prof_reads_first": s[21], "prof_reads_second": s[22]}
+                "prof_sibling_moves": s[20], "prof_reads_first": s[21], "prof_reads_second": s[22],
+                "lookahead_starts": s[23]}
