@@ -331,17 +331,29 @@ def main():
     out["lb"] = args.lb
     if learned is not None:
         out["learned_lb"] = learned
+    out["lookahead"] = bool(solver.config()["lookahead"]) and args.lb != "prevload"
     if not args.no_ablation and not per_instance:
-        # the same batch under the other dispatch orders (one step each)
-        for name, pol in (("uniform", spz.LB_UNIFORM), ("prevload", spz.LB_PREVLOAD)):
+        # the same batch under the other dispatch orders (one step each; the
+        # previous-load order runs one iteration at a time, without lookahead)
+        pols = {"uniform": spz.LB_UNIFORM, "learned": spz.LB_LEARNED, "prevload": spz.LB_PREVLOAD}
+        for name in ("uniform", "prevload"):
             if name == args.lb:
                 continue
-            solver.set_lb(pol)
+            solver.set_lb(pols[name])
             t = time.perf_counter()
             r2 = solver.solve(boards, max_iters=args.max_iters)
             dt = time.perf_counter() - t
             assert [(r.opt_len, r.counts) for r in r2] == [(r.opt_len, r.counts) for r in ref]
             out[f"{name}_lb_instances_per_s"] = len(boards) / dt
+        # the timed solver's order, iterations strictly one at a time
+        solver.set_lb(pols[args.lb])
+        solver.reconfigure(lookahead=0)
+        t = time.perf_counter()
+        r2 = solver.solve(boards, max_iters=args.max_iters)
+        dt = time.perf_counter() - t
+        assert [(r.opt_len, r.counts) for r in r2] == [(r.opt_len, r.counts) for r in ref]
+        out["no_lookahead_instances_per_s"] = len(boards) / dt
+        solver.reconfigure(lookahead=1)
     if pdb_ms is not None:
         out["pdb_build_ms"] = pdb_ms
         _, gps = solver.gather_probe(1 << 30)
@@ -363,6 +375,16 @@ def main():
         assert [(r.opt_len, r.counts) for r in pres] == [(r.opt_len, r.counts) for r in ref]
         out["plain_instances_per_s"] = n / pt
         out["speedup_vs_plain"] = inst_per_s / (n / pt)
+        if not args.no_ablation:
+            # the plain method with lookahead added (not the reference's method;
+            # shows what the batch adds beyond lookahead)
+            plain.reconfigure(lookahead=1)
+            t = time.perf_counter()
+            pres = solve_step(plain, boards, True, args.max_iters)
+            pt = time.perf_counter() - t
+            assert [(r.opt_len, r.counts) for r in pres] == [(r.opt_len, r.counts) for r in ref]
+            out["plain_lookahead_instances_per_s"] = n / pt
+            out["speedup_vs_plain_lookahead"] = inst_per_s / (n / pt)
         plain.close()
 
     if not args.no_cpu:

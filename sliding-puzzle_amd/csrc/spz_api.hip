@@ -87,7 +87,7 @@ struct spz_ctx {
     size_t qcap = 0;
     // per instance
     DBuf roots, L, it, cnt, pend, found, done, opt, iter_counts, restart_list;
-    DBuf itn, drained, nlive, Lbase, listed, want_next; // iteration slots (lookahead)
+    DBuf itn, drained, dead, nlive, Lbase, listed, want_next; // iteration slots (lookahead)
     DBuf frontier, nfront, topgen, top_goal, frozen, lvl[2], nlvl[2];
     // load balancing
     DBuf c_origin, load, samples, nsamp, pred;
@@ -392,7 +392,7 @@ void spz_destroy(spz_ctx *ctx) {
                    &ctx->pred,        &ctx->qpath,          &ctx->c_ipath,        &ctx->fpath,
                    &ctx->lpath[0],    &ctx->lpath[1],       &ctx->sol,            &ctx->sol_len,
                    &ctx->itn,         &ctx->drained,        &ctx->nlive,          &ctx->Lbase,
-                   &ctx->listed,      &ctx->want_next};
+                   &ctx->listed,      &ctx->want_next,      &ctx->dead};
     for (DBuf *b : all) b->release();
     if (ctx->mlp_alloced) mlp_free(ctx->mlp);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -521,6 +521,7 @@ static KParams base_params(spz_ctx *ctx, int n, int max_iters, int capl) {
     p.it = ctx->it.as<int>();
     p.itn = ctx->itn.as<int>();
     p.drained = ctx->drained.as<int>();
+    p.dead = ctx->dead.as<int>();
     p.nlive = ctx->nlive.as<int>();
     p.Lbase = ctx->Lbase.as<int>();
     p.listed = ctx->listed.as<int>();
@@ -635,6 +636,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     HIPCHK(ctx->pend.ensure(2 * N * 4));
     HIPCHK(ctx->found.ensure(2 * N * 4));
     HIPCHK(ctx->drained.ensure(2 * N * 4));
+    HIPCHK(ctx->dead.ensure(2 * N * 4));
     HIPCHK(ctx->it.ensure(N * 4));
     HIPCHK(ctx->nlive.ensure(N * 4));
     HIPCHK(ctx->Lbase.ensure(N * 4));
@@ -753,6 +755,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         HIPCHK(hipMemsetAsync(ctx->cnt.p, 0, 2 * N * 8, s));
         HIPCHK(hipMemsetAsync(ctx->pend.p, 0, 2 * N * 4, s));
         HIPCHK(hipMemsetAsync(ctx->drained.p, 0, 2 * N * 4, s));
+        HIPCHK(hipMemsetAsync(ctx->dead.p, 0, 2 * N * 4, s));
         HIPCHK(hipMemcpyAsync(ctx->found.p, nog.data(), 2 * N * 4, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemsetAsync(ctx->it.p, 0, N * 4, s));
         HIPCHK(hipMemsetAsync(ctx->nlive.p, 0, N * 4, s));

@@ -75,6 +75,8 @@ struct KParams {
     unsigned int *pend;          // [2 n_inst] its items not finished
     int *found;                  // [2 n_inst] min goal depth (kNoGoal if none)
     int *drained;                // [2 n_inst] 1 once pend reached zero
+    int *dead;                   // [2 n_inst] 1: the slot's work is not needed
+                                 // (instance done, or a goal in the older slot)
     int *it;                     // [n_inst] current (oldest unresolved) iteration
     int *nlive;                  // [n_inst] iterations started, not resolved (0-2)
     const int *Lbase;            // [n_inst] threshold of iteration 0 (h of the root)

@@ -57,13 +57,12 @@ __device__ void iteration_drained(const KParams &p, int isl) {
     }
 }
 
-// A lookahead iteration whose work is no longer needed: the instance is
-// done, or its older iteration (the other slot, lower index) has found a
-// goal and so is the last one.  Lanes drop such items unsearched.
-__device__ __forceinline__ bool superseded(const KParams &p, int isl) {
-    if (p.done[isl >> 1]) return true;
+// A goal in slot isl makes a newer iteration in the other slot unnecessary
+// (this one is the instance's last): mark that slot dead.  Lanes drop items
+// of a dead slot unsearched.
+__device__ __forceinline__ void kill_newer(const KParams &p, int isl) {
     const int o = isl ^ 1;
-    return p.found[o] != kNoGoal && p.itn[o] < p.itn[isl];
+    if (p.itn[o] > p.itn[isl]) p.dead[o] = 1;
 }
 
 // Append one item.  pos is a reserved queue position.
@@ -247,7 +246,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
             s.hs.B = (int)((v1 & 0xff) + ((v1 >> 8) & 0xff) + ((v1 >> 16) & 0xff) + (v1 >> 24));
         }
         L = p.L[inst];
-        if (p.lookahead && superseded(p, inst)) sp = 0, st = ST_POP; // unwinds and finishes next step
+        if (p.lookahead && p.dead[inst]) sp = 0, st = ST_POP; // unwinds and finishes next step
         for (int k = 0; k < sp; ++k) stk[k * sstride] = cstk[(size_t)k * NL + gl];
     } else {
         st = ST_IDLE;
@@ -267,7 +266,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
         cnt = 0;
         st = ST_FRESH;
         if (mode == 1 && p.found[inst] != kNoGoal) st = ST_POP; // first-solution: abandon
-        if (p.lookahead && superseded(p, inst)) st = ST_POP;
+        if (p.lookahead && p.dead[inst]) st = ST_POP;
     };
 
     // move down into child d (stack level sl, R = passing children still to
@@ -470,6 +469,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
             bool xp = true;
             if (h == 0) {
                 atomicMin(&p.found[inst], g);
+                if (p.lookahead) kill_newer(p, inst);
                 // the first goal of the slot's iteration records its path
                 // (the first iteration with a goal has threshold = optimum,
                 // and every goal it finds has that depth)
@@ -796,6 +796,8 @@ __device__ int resolve_instance(const KParams &p, int inst) {
         if (p.found[isc] != kNoGoal || cur + 1 >= p.max_iters) {
             p.opt[inst] = p.found[isc] != kNoGoal ? p.found[isc] : -1;
             p.done[inst] = 1;
+            p.dead[inst * 2] = 1;
+            p.dead[inst * 2 + 1] = 1;
             atomicSub(&p.ctl[1], 1u);
             return -1;
         }
@@ -865,6 +867,7 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
             p.cnt[isl] = s_cnt;
             p.pend[isl] = off;
             p.drained[isl] = off == 0 ? 1 : 0;
+            p.dead[isl] = 0;
         }
         __syncthreads();
         for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) {
