@@ -130,11 +130,18 @@ def train_predictor(solver, config, width, n_train=None):
             "setup_s": time.perf_counter() - t0}
 
 
+def progress(*a):
+    """Progress of the run on stderr (the JSON line alone goes to stdout)."""
+    print("bench:", *a, file=sys.stderr, flush=True)
+
+
 def solve_step(solver, boards, per_instance: bool, max_iters: int):
     if per_instance:
         out = []
-        for b in boards:
+        for i, b in enumerate(boards):
             out += solver.solve([b], max_iters=max_iters)
+            if (i + 1) % 512 == 0:
+                progress(f"{i + 1} of {len(boards)} instances one per launch")
         return out
     return solver.solve(boards, max_iters=max_iters)
 
@@ -255,6 +262,7 @@ def main():
     learned = None
     if args.lb == "learned":
         learned = train_predictor(solver, args.config, width)
+        progress("predictor trained")
         solver.set_lb(spz.LB_LEARNED)
     elif args.lb == "prevload":
         solver.set_lb(spz.LB_PREVLOAD)
@@ -262,6 +270,7 @@ def main():
     ref = None
     for _ in range(args.warmup):
         ref = solve_step(solver, boards, per_instance, args.max_iters)
+        progress("warmup step done")
     round_ms, launches, nodes_total = 0.0, 0, 0
     brk = {"solve_ms": 0.0, "setup_ms": 0.0, "split_ms": 0.0, "rounds_wall_ms": 0.0, "rounds": 0.0,
            "donated": 0.0}
@@ -277,6 +286,7 @@ def main():
         if ref is not None:
             assert [(r.opt_len, r.counts) for r in res] == [(r.opt_len, r.counts) for r in ref]
         ref = res
+        progress("timed step done")
     elapsed = time.perf_counter() - t0
     n = len(boards)
     ms_per_step = elapsed / args.steps * 1e3
@@ -345,6 +355,7 @@ def main():
             dt = time.perf_counter() - t
             assert [(r.opt_len, r.counts) for r in r2] == [(r.opt_len, r.counts) for r in ref]
             out[f"{name}_lb_instances_per_s"] = len(boards) / dt
+            progress(f"{name} order done")
         # the timed solver's order, iterations strictly one at a time
         solver.set_lb(pols[args.lb])
         solver.reconfigure(lookahead=0)
@@ -353,6 +364,7 @@ def main():
         dt = time.perf_counter() - t
         assert [(r.opt_len, r.counts) for r in r2] == [(r.opt_len, r.counts) for r in ref]
         out["no_lookahead_instances_per_s"] = len(boards) / dt
+        progress("no-lookahead solve done")
         solver.reconfigure(lookahead=1)
     if pdb_ms is not None:
         out["pdb_build_ms"] = pdb_ms
@@ -374,6 +386,7 @@ def main():
         pt = time.perf_counter() - t
         assert [(r.opt_len, r.counts) for r in pres] == [(r.opt_len, r.counts) for r in ref]
         out["plain_instances_per_s"] = n / pt
+        progress("plain pass done")
         out["speedup_vs_plain"] = inst_per_s / (n / pt)
         if not args.no_ablation:
             # the plain method with lookahead added (not the reference's method;
