@@ -317,8 +317,8 @@ __device__ __forceinline__ void do_move(Pos<W, HEUR, BD> &s, int d, const PdbCtx
 // do_move for the PDB state with the moved tile's two lookup values known
 template <int W, class BD>
 __device__ __forceinline__ void do_move_known(Pos<W, 1, BD> &s, int d, const PdbCtx &P, uint32_t vals) {
-    int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
-    int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
+    int nr = s.br + dir_dr(d);
+    int nc = s.bc + dir_dc(d);
     int from = nr * W + nc, to = s.br * W + s.bc;
     int t = s.bd.tile(from);
     pdb_apply_known(P, s.hs, t, from, to, vals);
@@ -333,8 +333,8 @@ __device__ __forceinline__ void do_move_known(Pos<W, 1, BD> &s, int d, const Pdb
 // move replaced, packed the same way, for the stack entry that undoes it.
 template <int W, class BD>
 __device__ __forceinline__ uint32_t move_pdb(Pos<W, 1, BD> &s, int d, const PdbCtx &P, bool known, uint32_t kv) {
-    const int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
-    const int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
+    const int nr = s.br + dir_dr(d);
+    const int nc = s.bc + dir_dc(d);
     const int from = nr * W + nc, to = s.br * W + s.bc;
     const int t = s.bd.tile(from);
     const PdbGeo &G = *P.geo;
@@ -374,8 +374,8 @@ struct PendMove {
 template <int W, class BD>
 __device__ __forceinline__ PendMove move_pdb_begin(Pos<W, 1, BD> &s, int d, const PdbCtx &P, bool known, uint32_t kv,
                                                    uint32_t &ov) {
-    const int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
-    const int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
+    const int nr = s.br + dir_dr(d);
+    const int nc = s.bc + dir_dc(d);
     const int from = nr * W + nc, to = s.br * W + s.bc;
     const int t = s.bd.tile(from);
     const PdbGeo &G = *P.geo;
@@ -411,8 +411,8 @@ __device__ __forceinline__ void move_pdb_finish(HState<1> &h, const PendMove &m)
 // blank move in direction d slides (packed regular | reflected << 8)
 template <int W, class BD>
 __device__ __forceinline__ uint32_t pdb_moved_vals(const Pos<W, 1, BD> &s, int d, const PdbCtx &P) {
-    int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
-    int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
+    int nr = s.br + dir_dr(d);
+    int nc = s.bc + dir_dc(d);
     int t = s.bd.tile(nr * W + nc);
     const PdbGeo &G = *P.geo;
     int kr = G.reg_k[t], kf = G.ref_k[t] - 4;

@@ -11,6 +11,16 @@ namespace spz {
 // (src_horie/idas_cpu.c:7-12).
 enum : int { DIR_UP = 0, DIR_RIGHT = 1, DIR_LEFT = 2, DIR_DOWN = 3, DIR_NONE = 4 };
 
+// Blank displacement of move d (up, right, left, down): row change, column
+// change and, on a width-W board, cell change, each the signed byte d of a
+// constant -- a shift and a sign extension, no branches.
+__device__ __forceinline__ int dir_byte(uint32_t packed, int d) { return (int)(int8_t)(packed >> (8 * d)); }
+__device__ __forceinline__ int dir_dr(int d) { return dir_byte(0x010000FFu, d); }
+__device__ __forceinline__ int dir_dc(int d) { return dir_byte(0x00FF0100u, d); }
+template <int W> __device__ __forceinline__ int dir_cell(int d) {
+    return dir_byte(((uint32_t)W << 24) | 0x00FF0100u | (uint32_t)(uint8_t)(-W), d);
+}
+
 // Lane states of the DFS engine.
 // lane states; ST_DESC (children f-tested, descent into the first passing one
 // pending) never outlives a round

@@ -443,7 +443,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                     if (j < ng) {
                         const int d = __builtin_ctz(rem);
                         rem &= rem - 1;
-                        const int from = bl + (d == DIR_UP ? -W : (d == DIR_DOWN ? W : (d == DIR_LEFT ? -1 : 1)));
+                        const int from = bl + dir_cell<W>(d);
                         const int t = s.bd.tile(from);
                         const uint2 wa = G.w[t][from], wb = G.w[t][bl];
                         const int kr = (int)(wa.x >> 28), kf = (int)(wa.y >> 28);
