@@ -628,7 +628,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     qcap = next_pow2(qcap);
     ctx->qcap = qcap;
 
-    HIPCHK(ctx->q.ensure(qcap * sizeof(Item)));
+    HIPCHK(ctx->q.ensure(2 * qcap * sizeof(Item))); // ring 0: current iterations, ring 1: lookahead
     HIPCHK(ctx->roots.ensure(N * sizeof(Item)));
     HIPCHK(ctx->L.ensure(2 * N * 4));
     HIPCHK(ctx->itn.ensure(2 * N * 4));
@@ -676,7 +676,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     if (paths) {
         if (ctx->cfg.max_split_depth > 32)
             return fail(ctx, SPZ_E_ARG, "record_paths needs max_split_depth <= 32");
-        HIPCHK(ctx->qpath.ensure(qcap * kPathWords * 4));
+        HIPCHK(ctx->qpath.ensure(2 * qcap * kPathWords * 4));
         HIPCHK(ctx->c_ipath.ensure((size_t)kPathWords * lanes * 4));
         HIPCHK(ctx->fpath.ensure(front_items * 8));
         for (int k = 0; k < 2; ++k) HIPCHK(ctx->lpath[k].ensure(front_items * 8));
@@ -797,8 +797,8 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         for (auto &x : ev) HIPCHK(hipEventCreate(&x));
     double round_ms = 0;
     long timed = 0;
-    unsigned *h_ctl = nullptr; // pinned: 64 control words, then the 3 queue counters
-    HIPCHK(hipHostMalloc((void **)&h_ctl, 64 * 4 + 3 * 8, hipHostMallocDefault));
+    unsigned *h_ctl = nullptr; // pinned: 64 control words, then the 2 x 3 queue counters
+    HIPCHK(hipHostMalloc((void **)&h_ctl, 64 * 4 + 6 * 8, hipHostMallocDefault));
     unsigned long long *h_qc = (unsigned long long *)(h_ctl + 64);
     int rc = SPZ_OK;
     long rounds = 0;
@@ -815,7 +815,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
             if (e == hipSuccess) e = launch_control(p, s);
         }
         if (e == hipSuccess) e = hipMemcpyAsync(h_ctl, ctx->ctl.p, 64 * 4, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(h_qc, ctx->q_ctl.p, 3 * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(h_qc, ctx->q_ctl.p, 6 * 8, hipMemcpyDeviceToHost, s);
         return e;
     };
     // hipGraph: the period is captured once and replayed; the kernels read
@@ -860,7 +860,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         if (trace) {
             fprintf(stderr, "spz poll %ld: %.1f ms, busy lanes %u, active instances %u, queued %llu", rounds,
                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_rounds).count(),
-                    h_ctl[12], h_ctl[1], (unsigned long long)(h_qc[2] - h_qc[0]));
+                    h_ctl[12], h_ctl[1], (unsigned long long)(h_qc[2] - h_qc[0] + h_qc[5] - h_qc[3]));
             unsigned long long pf[4];
             if (p.prof && hipMemcpy(pf, p.prof, sizeof pf, hipMemcpyDeviceToHost) == hipSuccess)
                 fprintf(stderr, ", lane-steps idle %llu expanding %llu popping %llu", pf[0], pf[1], pf[2]);
@@ -874,7 +874,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         // no lane busy at the end of the last round and nothing queued (a
         // finished iteration has queued its next one by now): no work can
         // appear any more
-        stalled = (h_qc[2] == h_qc[0] && h_ctl[12] == 0) ? stalled + 1 : 0;
+        stalled = (h_qc[2] == h_qc[0] && h_qc[5] == h_qc[3] && h_ctl[12] == 0) ? stalled + 1 : 0;
         if (stalled > 2) {
             rc = fail(ctx, SPZ_E_STALL, "search stalled with %u active instances", h_ctl[1]);
             break;

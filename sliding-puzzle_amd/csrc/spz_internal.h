@@ -65,7 +65,9 @@ __host__ __device__ inline int meta_prev(uint32_t m) { return (int)(m >> 24); }
 
 // Everything the kernels touch, passed by value.
 struct KParams {
-    // work queue: ring of qcap (power of two) items; q_ctl = head, limit, tail
+    // work queue: two rings of qcap (power of two) items each, ring 0 for
+    // the current iterations, ring 1 for lookahead iterations (ring r at
+    // q + r * qcap); q_ctl[3r..3r+2] = head, limit, tail of ring r
     Item *q;
     unsigned long long qmask;
     unsigned long long *q_ctl;
@@ -101,7 +103,9 @@ struct KParams {
                                  // 4 donate_per_lane, 5 budget, 6 rounds,
                                  // 7 donated, 8 overflow exports, 9/10 big/small
                                  // budget, 12 busy at last round end,
-                                 // 13 lookahead iterations started
+                                 // 13 lookahead iterations started,
+                                 // 18 busy lanes on lookahead iterations,
+                                 // 19 donate_per_lane of those lanes
 
     // top tree of the root split (per instance)
     const Item *frontier;        // [n_inst][capl]
@@ -140,7 +144,7 @@ struct KParams {
 
     // solution paths (record_paths; all null when off): moves of the blank
     // from the instance root, 2 bits each, kPathWords words per path
-    uint32_t *qpath;                 // [qcap][kPathWords] paths of the queued items
+    uint32_t *qpath;                 // [2][qcap][kPathWords] paths of the queued items
     uint32_t *c_ipath;               // [kPathWords][n_lanes] path of each lane's item root
     const unsigned long long *fpath; // [n_inst][capl] paths of the frontier nodes
     uint32_t *sol;                   // [2 n_inst][kPathWords] one goal path per slot

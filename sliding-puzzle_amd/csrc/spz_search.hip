@@ -65,28 +65,42 @@ __device__ __forceinline__ void kill_newer(const KParams &p, int isl) {
     if (p.itn[o] > p.itn[isl]) p.dead[o] = 1;
 }
 
-// Append one item.  pos is a reserved queue position.
+// The work queue is two rings of equal size: ring 0 holds work of the
+// instances' current iterations, ring 1 that of lookahead iterations; idle
+// lanes take from ring 0 first.  Slot of position pos of ring r:
+__device__ __forceinline__ size_t q_slot(const KParams &p, int r, unsigned long long pos) {
+    return (size_t)r * (size_t)(p.qmask + 1ull) + (size_t)(pos & p.qmask);
+}
+
+// Does the slot index isl belong to a lookahead iteration (newer than the
+// instance's current one)?  Its work then goes to ring 1.
+__device__ __forceinline__ int ahead_ring(const KParams &p, int isl) {
+    return (p.lookahead && p.itn[isl] > p.it[isl >> 1]) ? 1 : 0;
+}
+
+// Append one item.  pos is a reserved position of ring r.
 template <int W, int HEUR, class BD>
-__device__ __forceinline__ void put_item(const KParams &p, unsigned long long pos, const Pos<W, HEUR, BD> &s, int g,
-                                         int prev, int inst, uint32_t origin) {
+__device__ __forceinline__ void put_item(const KParams &p, int r, unsigned long long pos, const Pos<W, HEUR, BD> &s,
+                                         int g, int prev, int inst, uint32_t origin) {
     Item it;
     s.bd.pack(it.b);
     it.meta = meta_pack(s.br * W + s.bc, g, s.hs.h, prev);
     it.inst = (uint32_t)inst;
     it.x0 = 0;
     it.x1 = origin;
-    p.q[pos & p.qmask] = it;
+    p.q[q_slot(p, r, pos)] = it;
 }
 
-// Reserve k slots at the queue tail.  Items in [min(head, limit), tail) are
-// not taken yet; a tail more than the ring's capacity past them would
-// overwrite some, so that sets the error word (ctl[3]) and the host fails the
-// solve with SPZ_E_CAPACITY rather than return counts from a corrupt queue.
-// head is read as it stands, possibly behind, which only makes the test
-// stricter.
-__device__ __forceinline__ unsigned long long q_reserve(const KParams &p, unsigned long long k) {
-    const unsigned long long pos = atomicAdd(&p.q_ctl[2], k);
-    const unsigned long long h = *(volatile unsigned long long *)&p.q_ctl[0], lim = p.q_ctl[1];
+// Reserve k slots at the tail of ring r (counters q_ctl[3r..3r+2]: head,
+// limit, tail).  Items in [min(head, limit), tail) are not taken yet; a tail
+// more than the ring's capacity past them would overwrite some, so that sets
+// the error word (ctl[3]) and the host fails the solve with SPZ_E_CAPACITY
+// rather than return counts from a corrupt queue.  head is read as it
+// stands, possibly behind, which only makes the test stricter.
+__device__ __forceinline__ unsigned long long q_reserve(const KParams &p, int r, unsigned long long k) {
+    unsigned long long *c = p.q_ctl + 3 * r;
+    const unsigned long long pos = atomicAdd(&c[2], k);
+    const unsigned long long h = *(volatile unsigned long long *)&c[0], lim = c[1];
     const unsigned long long live_from = h < lim ? h : lim;
     if (pos + k - live_from > p.qmask + 1ull) atomicOr(&p.ctl[3], 1u);
     return pos;
@@ -176,7 +190,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
         __syncthreads();
     }
 
-    const unsigned long long limit = p.q_ctl[1];
+    const unsigned long long limit = p.q_ctl[1], limit1 = p.q_ctl[4];
     const int budget = (int)p.ctl[5];
     const int mode = p.mode;
 
@@ -286,8 +300,8 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     };
     // path of a new queue item: the item root's path, the moves of stack
     // levels 0..nst-1, then move last_d
-    auto item_path = [&](unsigned long long qpos, int nst, int last_d) {
-        const uint32_t qo = (uint32_t)(qpos & p.qmask) * kPathWords;
+    auto item_path = [&](int r, unsigned long long qpos, int nst, int last_d) {
+        const uint32_t qo = (uint32_t)q_slot(p, r, qpos) * kPathWords;
         path_copy(p.c_ipath, (uint32_t)gl, (uint32_t)NL, p.qpath, qo, 1);
         const int gr = meta_g(rmeta);
         for (int k = 0; k < nst; ++k) path_put(p.qpath, qo, 1, gr + k, SE::dir(stk[k * sstride]));
@@ -313,12 +327,32 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                 if (st == ST_IDLE) {
                     unsigned long long my = base + __popcll(need & lanemask_lt());
                     if (my < limit) {
-                        start_item(p.q[my & p.qmask]);
+                        start_item(p.q[q_slot(p, 0, my)]);
                         if (p.qpath)
-                            path_copy(p.qpath, (uint32_t)(my & p.qmask) * kPathWords, 1, p.c_ipath, (uint32_t)gl,
+                            path_copy(p.qpath, (uint32_t)q_slot(p, 0, my) * kPathWords, 1, p.c_ipath, (uint32_t)gl,
                                       (uint32_t)NL);
-                    } else {
+                    } else if (!p.lookahead) {
                         st = ST_DRAINED;
+                    }
+                }
+                // lanes ring 0 could not serve try the lookahead ring
+                const uint64_t need1 = p.lookahead ? __ballot(st == ST_IDLE) : 0ull;
+                if (need1) {
+                    const int n1 = __popcll(need1);
+                    const int leader1 = __ffsll((long long)need1) - 1;
+                    unsigned long long base1 = 0;
+                    if (lane == leader1) base1 = atomicAdd(&p.q_ctl[3], (unsigned long long)n1);
+                    base1 = shfl_u64(base1, leader1);
+                    if (st == ST_IDLE) {
+                        const unsigned long long my = base1 + __popcll(need1 & lanemask_lt());
+                        if (my < limit1) {
+                            start_item(p.q[q_slot(p, 1, my)]);
+                            if (p.qpath)
+                                path_copy(p.qpath, (uint32_t)q_slot(p, 1, my) * kPathWords, 1, p.c_ipath,
+                                          (uint32_t)gl, (uint32_t)NL);
+                        } else {
+                            st = ST_DRAINED;
+                        }
                     }
                 }
             }
@@ -562,15 +596,16 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                     // stack full: the passing children become queue items
                     int k = __builtin_popcount(pass);
                     atomicAdd(&p.pend[inst], (unsigned)k);
-                    unsigned long long pos = q_reserve(p, (unsigned long long)k);
+                    const int r = ahead_ring(p, inst);
+                    unsigned long long pos = q_reserve(p, r, (unsigned long long)k);
                     atomicAdd(&p.ctl[8], (unsigned)k);
                     while (pass) {
                         int d = __builtin_ctz(pass);
                         pass &= pass - 1;
-                        if (p.qpath) item_path(pos, sp, d);
+                        if (p.qpath) item_path(r, pos, sp, d);
                         Pos<W, HEUR> c = reg_pos(s);
                         do_move<W, HEUR>(c, d, P);
-                        put_item<W, HEUR>(p, pos++, c, g + 1, d, inst, origin);
+                        put_item<W, HEUR>(p, r, pos++, c, g + 1, d, inst, origin);
                     }
                     st = ST_POP;
                 }
@@ -606,7 +641,9 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
 
     // ---- round end: donate shallow open nodes, then save the context ----
     const bool busy = (st == ST_FRESH || st == ST_POP);
-    const int per_lane = (int)p.ctl[4];
+    // lanes on lookahead iterations donate into ring 1, under their own quota
+    const int ring = busy ? ahead_ring(p, inst) : 0;
+    const int per_lane = (int)(ring ? p.ctl[19] : p.ctl[4]);
     int want = 0;
     if (busy && per_lane > 0) {
         int avail = 0;
@@ -614,22 +651,27 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
         want = avail < per_lane ? avail : per_lane;
     }
     if (__ballot(want > 0)) {
-        // wave prefix sum of `want`, one queue reservation per wave
-        int incl = want;
+        // per ring: wave prefix sum of `want`, one queue reservation per wave
+        unsigned long long pos = 0;
+        for (int r = 0; r < 2; ++r) {
+            const int wr = ring == r ? want : 0;
+            if (!__ballot(wr > 0)) continue;
+            int incl = wr;
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            int v = __shfl_up(incl, off);
-            if (lane >= off) incl += v;
+            for (int off = 1; off < 64; off <<= 1) {
+                int v = __shfl_up(incl, off);
+                if (lane >= off) incl += v;
+            }
+            int total = __shfl(incl, 63);
+            unsigned long long base = 0;
+            if (lane == 0) {
+                base = q_reserve(p, r, (unsigned long long)total);
+                atomicAdd(&p.ctl[7], (unsigned)total);
+            }
+            base = shfl_u64(base, 0);
+            if (wr > 0) pos = base + (unsigned long long)(incl - wr);
         }
-        int total = __shfl(incl, 63);
-        unsigned long long base = 0;
-        if (lane == 0) {
-            base = q_reserve(p, (unsigned long long)total);
-            atomicAdd(&p.ctl[7], (unsigned)total);
-        }
-        base = shfl_u64(base, 0);
         if (want > 0) {
-            unsigned long long pos = base + (unsigned long long)(incl - want);
             atomicAdd(&p.pend[inst], (unsigned)want);
             if (p.qpath) {
                 // the donated items' paths first, in the order of the pass
@@ -642,7 +684,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                     while (R && emitted < want) {
                         const int d2 = __builtin_ctz(R);
                         R &= R - 1;
-                        item_path(pp++, k, d2);
+                        item_path(ring, pp++, k, d2);
                         ++emitted;
                     }
                 }
@@ -661,7 +703,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                     R &= R - 1;
                     Pos<W, HEUR> x = c;
                     do_move<W, HEUR>(x, d2, P);
-                    put_item<W, HEUR>(p, pos++, x, cg + 1, d2, inst, origin);
+                    put_item<W, HEUR>(p, ring, pos++, x, cg + 1, d2, inst, origin);
                     ++emitted;
                 }
                 stk[k * sstride] = SE::make(d, R, SE::vals(e));
@@ -701,6 +743,8 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     }
     uint64_t busy_mask = __ballot(busy);
     if (lane == 0 && busy_mask) atomicAdd(&p.ctl[2], (unsigned)__popcll(busy_mask));
+    const uint64_t ahead_mask = p.lookahead ? __ballot(busy && ring == 1) : 0ull;
+    if (lane == 0 && ahead_mask) atomicAdd(&p.ctl[18], (unsigned)__popcll(ahead_mask));
     if (busy) {
         p.c_st[gl] = (uint32_t)st | ((uint32_t)sp << 8);
         uint32_t b[4];
@@ -810,7 +854,7 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
     __shared__ unsigned long long s_cnt;
     __shared__ unsigned s_bucket[64];
     __shared__ unsigned long long s_base;
-    __shared__ int s_j;
+    __shared__ int s_j, s_ring;
     const unsigned n_list = p.ctl[0];
     if (blockIdx.x >= n_list) return;
     const int inst = p.restart_list[blockIdx.x];
@@ -819,7 +863,10 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
     unsigned long long *load = p.load + (size_t)inst * p.capl;
 
     for (;;) {
-        if (threadIdx.x == 0) s_j = resolve_instance(p, inst);
+        if (threadIdx.x == 0) {
+            s_j = resolve_instance(p, inst);
+            s_ring = s_j > p.it[inst] ? 1 : 0; // a lookahead start goes to ring 1
+        }
         __syncthreads();
         const int it = s_j;
         if (it < 0) break;
@@ -860,7 +907,7 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
                 s_bucket[b] = off;
                 off += n;
             }
-            s_base = off ? q_reserve(p, (unsigned long long)off) : 0ull;
+            s_base = off ? q_reserve(p, s_ring, (unsigned long long)off) : 0ull;
             p.L[isl] = L;
             p.itn[isl] = it;
             p.found[isl] = p.top_goal[inst] <= L ? p.top_goal[inst] : kNoGoal;
@@ -876,10 +923,9 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
                 unsigned k = atomicAdd(&s_bucket[lb_bucket(p, inst, i, it)], 1u);
                 x.inst = (uint32_t)isl;
                 x.x1 = i;
-                p.q[(s_base + k) & p.qmask] = x;
-                if (p.qpath)
-                    path_from_u64(p.fpath[(size_t)inst * p.capl + i], p.qpath,
-                                  (uint32_t)((s_base + k) & p.qmask) * kPathWords);
+                const size_t slot = q_slot(p, s_ring, s_base + k);
+                p.q[slot] = x;
+                if (p.qpath) path_from_u64(p.fpath[(size_t)inst * p.capl + i], p.qpath, (uint32_t)slot * kPathWords);
             }
         }
         // 4. loads of the new iteration start from zero
@@ -901,8 +947,11 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
 __global__ void __launch_bounds__(256) k_spec(KParams p) {
     __shared__ unsigned long long s_room, s_used;
     if (threadIdx.x == 0) {
-        const unsigned long long head = p.q_ctl[0], lim = p.q_ctl[1], tail = p.q_ctl[2];
-        const unsigned long long qlen = tail - (head < lim ? head : lim);
+        unsigned long long qlen = 0;
+        for (int r = 0; r < 2; ++r) {
+            const unsigned long long head = p.q_ctl[3 * r], lim = p.q_ctl[3 * r + 1], tail = p.q_ctl[3 * r + 2];
+            qlen += tail - (head < lim ? head : lim);
+        }
         const unsigned long long lanes = (unsigned long long)p.n_lanes, busy = p.ctl[2];
         const unsigned long long taken = busy + qlen;
         s_room = (taken < lanes && lanes - taken >= lanes / 32) ? lanes - taken : 0ull;
@@ -929,34 +978,44 @@ __global__ void __launch_bounds__(256) k_spec(KParams p) {
     }
 }
 
-// k_control: between rounds.  Publishes the items appended during the last
-// round, sizes the donation quota from the number of idle lanes, and picks
-// the next round's step budget.  ctl[9] = budget used when the queue can keep
-// every lane busy, ctl[10] = budget while it cannot.
+// k_control: between rounds.  Publishes the items appended to both rings
+// during the last round, sizes the donation quotas from the number of idle
+// lanes, and picks the next round's step budget.  ctl[9] = budget used when
+// the queue can keep every lane busy, ctl[10] = budget while it cannot.
+// Lanes on current iterations (ctl[4]) donate to fill the idle lanes not
+// covered by ring 0; lanes on lookahead iterations (ctl[19]) only where both
+// rings together fall short, and their items wait behind ring 0's.
+__device__ __forceinline__ unsigned donate_quota(const KParams &p, unsigned long long idle,
+                                                 unsigned long long queued, unsigned long long donors) {
+    if (queued >= idle || donors == 0) return 0;
+    unsigned long long per = (idle - queued + donors - 1) / donors;
+    // donations of one round stay below a quarter of a ring
+    const unsigned long long cap = ((p.qmask + 1ull) / 4ull) / donors;
+    if (per > cap) per = cap;
+    return (unsigned)(per > 32ull ? 32ull : per);
+}
+
 __global__ void k_control(KParams p) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    unsigned long long head = p.q_ctl[0], limit = p.q_ctl[1], tail = p.q_ctl[2];
-    if (head > limit) head = limit;
-    p.q_ctl[0] = head;
-    p.q_ctl[1] = tail;
-    unsigned long long qlen = tail - head;
-    unsigned busy = p.ctl[2];
-    p.ctl[2] = 0;
-    p.ctl[12] = busy; // lanes busy at the end of the last round, for the host
-    p.ctl[0] = 0; // restart list consumed
-    unsigned long long lanes = (unsigned long long)p.n_lanes;
-    unsigned long long idle = lanes > busy ? lanes - busy : 0ull;
-    unsigned per_lane = 0;
-    if (qlen < idle && busy > 0) {
-        unsigned long long want = idle - qlen;
-        per_lane = (unsigned)((want + busy - 1) / busy);
-        // donations of one round stay below a quarter of the ring
-        unsigned long long cap = ((p.qmask + 1ull) / 4ull) / (unsigned long long)busy;
-        if (per_lane > cap) per_lane = (unsigned)cap;
-        if (per_lane > 32u) per_lane = 32u;
+    unsigned long long qlen[2];
+    for (int r = 0; r < 2; ++r) {
+        unsigned long long *c = p.q_ctl + 3 * r;
+        unsigned long long head = c[0], limit = c[1], tail = c[2];
+        if (head > limit) head = limit;
+        c[0] = head;
+        c[1] = tail;
+        qlen[r] = tail - head;
     }
-    p.ctl[4] = per_lane;
-    p.ctl[5] = (qlen + busy >= lanes / 2) ? p.ctl[9] : p.ctl[10];
+    const unsigned busy = p.ctl[2], busy1 = p.ctl[18];
+    p.ctl[2] = 0;
+    p.ctl[18] = 0;
+    p.ctl[12] = busy; // lanes busy at the end of the last round, for the host
+    p.ctl[0] = 0;     // restart list consumed
+    const unsigned long long lanes = (unsigned long long)p.n_lanes;
+    const unsigned long long idle = lanes > busy ? lanes - busy : 0ull;
+    p.ctl[4] = donate_quota(p, idle, qlen[0], busy - busy1);
+    p.ctl[19] = donate_quota(p, idle, qlen[0] + qlen[1], busy1);
+    p.ctl[5] = (qlen[0] + qlen[1] + busy >= lanes / 2) ? p.ctl[9] : p.ctl[10];
     p.ctl[6] += 1;
 }
 
