@@ -85,7 +85,9 @@ typedef struct spz_config {
     int32_t log_samples;     /* >0: keep up to this many (frontier node,
                                 threshold, subtree count) samples per solve
                                 (spz_get_samples); training data of the MLP */
-    int64_t queue_items;     /* work-queue capacity in items (0 = auto) */
+    int64_t queue_items;     /* capacity in items of each of the two work-queue
+                                rings (current iterations, lookahead), rounded
+                                up to a power of two (0 = auto) */
     int32_t record_paths;    /* 1: record one optimal solution path per instance
                                 (spz_get_paths); needs max_split_depth <= 32 */
     int32_t lookahead;       /* 1 (default): while lanes are idle, start an
