@@ -14,7 +14,8 @@
 #include <vector>
 
 namespace spz {
-hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo, int wg, hipStream_t s);
+hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo, int wg, hipStream_t s,
+                        hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 size_t round_lds_bytes(int width, int maxd, int wg, int heur);
 const void *round_kernel_ptr(int width, int heur);
 hipError_t launch_restart(const KParams &p, hipStream_t s);
@@ -807,9 +808,10 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     auto enqueue_period = [&]() -> hipError_t {
         hipError_t e = hipSuccess;
         for (int k = 0; k < per_poll && e == hipSuccess; ++k) {
-            if (timing) e = hipEventRecord(ev[2 * k], s);
-            if (e == hipSuccess) e = launch_round(w, heur, p, ctx->geo_d.as<PdbGeo>(), ctx->wg, s);
-            if (timing && e == hipSuccess) e = hipEventRecord(ev[2 * k + 1], s);
+            // timing: the runtime records the events with the dispatch
+            if (e == hipSuccess)
+                e = launch_round(w, heur, p, ctx->geo_d.as<PdbGeo>(), ctx->wg, s, timing ? ev[2 * k] : nullptr,
+                                 timing ? ev[2 * k + 1] : nullptr);
             if (e == hipSuccess && p.lookahead) e = launch_spec(p, s);
             if (e == hipSuccess) e = launch_restart(p, s);
             if (e == hipSuccess) e = launch_control(p, s);

@@ -11,12 +11,18 @@
 //     shallowest unexplored children back to the queue when lanes go idle
 //     (device-wide work stealing; the reference steals only inside a block,
 //     src_horie/idas_dynamic.cu:199-247).
-// The count of iteration L is  sum_{top u: f(u)<=L} gen(u)
-//                            + sum_{frontier v: f(v)<=L} subtree_count(v),
+// The count of iteration L is  sum_{top u: F(u)<=L} gen(u)
+//                            + sum_{frontier v: F(v)<=L} subtree_count(v),
 // where gen(u) counts every in-bounds non-reverse child of an expanded node
-// before its f-test (src/24puzzle.cc:119-131).  f is monotone along a path
-// (Manhattan and the PDB max are consistent), so a node is expanded in
-// iteration L iff its own f <= L, whatever the schedule: counts are exact.
+// before its f-test (src/24puzzle.cc:119-131) and F(u) is the largest f on
+// u's path from the root.  IDA* expands a node in iteration L iff every node
+// on its path has f <= L, i.e. F <= L; for Manhattan (consistent) F is the
+// node's own f, the PDB max is not consistent, hence the path maximum.  So
+// the counts are exact whatever the schedule.  With lookahead an instance
+// runs its next iteration beside the current one; each keeps its own count
+// and k_restart resolves them in order (k_spec, k_restart).
+#include <hip/hip_ext.h>
+
 #include "spz_board.h"
 
 namespace spz {
@@ -1148,16 +1154,24 @@ __global__ void k_scatter_roots(const Item *roots, Item *lvl, int capl, int n) {
 size_t round_lds_bytes(int width, int maxd, int wg, int heur);
 
 template <int W, int HEUR>
-static hipError_t launch_round_t(const KParams &p, const PdbGeo *geo, int wg, hipStream_t s) {
+// With t0/t1 the runtime records the two events with the dispatch itself, so
+// their interval is the kernel's own duration (no launch gap before it).
+static hipError_t launch_round_t(const KParams &p, const PdbGeo *geo, int wg, hipStream_t s, hipEvent_t t0,
+                                 hipEvent_t t1) {
     size_t lds = round_lds_bytes(W, p.maxd, wg, HEUR);
-    hipLaunchKernelGGL((k_round<W, HEUR>), dim3(p.n_lanes / wg), dim3(wg), lds, s, p, geo);
+    if (t0 && t1)
+        hipExtLaunchKernelGGL((k_round<W, HEUR>), dim3(p.n_lanes / wg), dim3(wg), (uint32_t)lds, s, t0, t1, 0u, p,
+                              geo);
+    else
+        hipLaunchKernelGGL((k_round<W, HEUR>), dim3(p.n_lanes / wg), dim3(wg), lds, s, p, geo);
     return hipGetLastError();
 }
 
-hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo, int wg, hipStream_t s) {
-    if (width == 4) return launch_round_t<4, 0>(p, geo, wg, s);
-    if (heur == 0) return launch_round_t<5, 0>(p, geo, wg, s);
-    return launch_round_t<5, 1>(p, geo, wg, s);
+hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo, int wg, hipStream_t s,
+                        hipEvent_t t0, hipEvent_t t1) {
+    if (width == 4) return launch_round_t<4, 0>(p, geo, wg, s, t0, t1);
+    if (heur == 0) return launch_round_t<5, 0>(p, geo, wg, s, t0, t1);
+    return launch_round_t<5, 1>(p, geo, wg, s, t0, t1);
 }
 
 // dynamic LDS of k_round: the stack levels, then (width 5) the boards, then
