@@ -80,6 +80,7 @@ def test_default_config_values():
     spz.lib().spz_default_config(ctypes.byref(cfg), 5, spz.H_PDB666)
     assert (cfg.width, cfg.heuristic, cfg.mode, cfg.frontier_target) == (5, 1, 0, 2048)
     assert cfg.workgroup_size == 256 and cfg.time_kernels == 0
+    assert cfg.lookahead == 1 and cfg.record_paths == 0 and cfg.use_graph == 1
 
 
 def test_config_layout_matches_c(tmp_path):
