@@ -47,7 +47,7 @@ struct DBuf {
     size_t n = 0;
     hipError_t ensure(size_t bytes) {
         if (bytes <= n) return hipSuccess;
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p); // the old contents are not needed; a failed free leaks at worst
         p = nullptr;
         n = 0;
         hipError_t e = hipMalloc(&p, bytes);
@@ -55,7 +55,7 @@ struct DBuf {
         return e;
     }
     void release() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
     }
@@ -270,7 +270,7 @@ int spz_create(spz_ctx **out, const spz_config *cfg) {
         pdb_geo_build(&ctx->geo_h);
         if ((e = ctx->geo_d.ensure(sizeof(PdbGeo))) != hipSuccess ||
             (e = hipMemcpy(ctx->geo_d.p, &ctx->geo_h, sizeof(PdbGeo), hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = ctx->q_ctl.ensure(64)) != hipSuccess || (e = ctx->ctl.ensure(64 * 4)) != hipSuccess) {
+            (e = ctx->q_ctl.ensure(2 * kQCtl * 8)) != hipSuccess || (e = ctx->ctl.ensure(64 * 4)) != hipSuccess) {
             rc = fail(ctx, SPZ_E_HIP, "context allocation: %s", hipGetErrorString(e));
             break;
         }
@@ -396,7 +396,7 @@ void spz_destroy(spz_ctx *ctx) {
                    &ctx->listed,      &ctx->want_next,      &ctx->dead};
     for (DBuf *b : all) b->release();
     if (ctx->mlp_alloced) mlp_free(ctx->mlp);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -775,7 +775,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         ctl[10] = (unsigned)std::min(big, 32);
         ctl[5] = ctl[10];
         HIPCHK(hipMemcpyAsync(ctx->ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemsetAsync(ctx->q_ctl.p, 0, 64, s));
+        HIPCHK(hipMemsetAsync(ctx->q_ctl.p, 0, 2 * kQCtl * 8, s));
         HIPCHK(hipMemsetAsync(ctx->c_st.p, 0, lanes * 4, s));
         HIPCHK(hipStreamSynchronize(s));
     }
@@ -798,8 +798,8 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         for (auto &x : ev) HIPCHK(hipEventCreate(&x));
     double round_ms = 0;
     long timed = 0;
-    unsigned *h_ctl = nullptr; // pinned: 64 control words, then the 2 x 3 queue counters
-    HIPCHK(hipHostMalloc((void **)&h_ctl, 64 * 4 + 6 * 8, hipHostMallocDefault));
+    unsigned *h_ctl = nullptr; // pinned: 64 control words, then the two rings' queue counters
+    HIPCHK(hipHostMalloc((void **)&h_ctl, 64 * 4 + 2 * kQCtl * 8, hipHostMallocDefault));
     unsigned long long *h_qc = (unsigned long long *)(h_ctl + 64);
     int rc = SPZ_OK;
     long rounds = 0;
@@ -817,7 +817,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
             if (e == hipSuccess) e = launch_control(p, s);
         }
         if (e == hipSuccess) e = hipMemcpyAsync(h_ctl, ctx->ctl.p, 64 * 4, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(h_qc, ctx->q_ctl.p, 6 * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(h_qc, ctx->q_ctl.p, 2 * kQCtl * 8, hipMemcpyDeviceToHost, s);
         return e;
     };
     // hipGraph: the period is captured once and replayed; the kernels read
@@ -834,8 +834,8 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         if (e2 == hipSuccess && e3 == hipSuccess) e3 = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
         if (e != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
             // capture unsupported here: run the same sequence eagerly
-            if (gexec) hipGraphExecDestroy(gexec);
-            if (graph) hipGraphDestroy(graph);
+            if (gexec) (void)hipGraphExecDestroy(gexec);
+            if (graph) (void)hipGraphDestroy(graph);
             gexec = nullptr;
             graph = nullptr;
             use_graph = false;
@@ -862,7 +862,8 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         if (trace) {
             fprintf(stderr, "spz poll %ld: %.1f ms, busy lanes %u, active instances %u, queued %llu", rounds,
                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_rounds).count(),
-                    h_ctl[12], h_ctl[1], (unsigned long long)(h_qc[2] - h_qc[0] + h_qc[5] - h_qc[3]));
+                    h_ctl[12], h_ctl[1],
+                    (unsigned long long)(h_qc[2] - h_qc[0] + h_qc[kQCtl + 2] - h_qc[kQCtl]));
             unsigned long long pf[4];
             if (p.prof && hipMemcpy(pf, p.prof, sizeof pf, hipMemcpyDeviceToHost) == hipSuccess)
                 fprintf(stderr, ", lane-steps idle %llu expanding %llu popping %llu", pf[0], pf[1], pf[2]);
@@ -876,7 +877,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         // no lane busy at the end of the last round and nothing queued (a
         // finished iteration has queued its next one by now): no work can
         // appear any more
-        stalled = (h_qc[2] == h_qc[0] && h_qc[5] == h_qc[3] && h_ctl[12] == 0) ? stalled + 1 : 0;
+        stalled = (h_qc[2] == h_qc[0] && h_qc[kQCtl + 2] == h_qc[kQCtl] && h_ctl[12] == 0) ? stalled + 1 : 0;
         if (stalled > 2) {
             rc = fail(ctx, SPZ_E_STALL, "search stalled with %u active instances", h_ctl[1]);
             break;
@@ -887,12 +888,12 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
             break;
         }
     }
-    if (gexec) hipGraphExecDestroy(gexec);
-    if (graph) hipGraphDestroy(graph);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (timing)
-        for (auto &x : ev) hipEventDestroy(x);
+        for (auto &x : ev) (void)hipEventDestroy(x);
     if (rc != SPZ_OK) {
-        hipHostFree(h_ctl);
+        (void)hipHostFree(h_ctl);
         return rc;
     }
     ctx->stats[14] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_rounds).count();
@@ -903,7 +904,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     ctx->stats[3] = (double)lanes;
     ctx->stats[7] = (double)h_ctl[8];
     ctx->stats[23] = (double)h_ctl[13];
-    hipHostFree(h_ctl);
+    (void)hipHostFree(h_ctl);
 
     std::vector<int> opt(N), its(N);
     std::vector<unsigned long long> ic(N * (size_t)max_iters);
@@ -983,8 +984,8 @@ int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, double *ms, double *gather
     if (e == hipSuccess) e = hipEventSynchronize(b);
     float t = 0;
     if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     out.release();
     if (e != hipSuccess) return fail(ctx, SPZ_E_HIP, "gather probe: %s", hipGetErrorString(e));
     *ms = t;
@@ -1111,7 +1112,7 @@ int spz_mlp_train(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const
             return fail(ctx, SPZ_E_ARG, "board %d is not a solvable permutation", i);
         pack_root(w, boards + (size_t)i * w * w, i, &it[i]);
     }
-    DBuf items, thr_all, tgt_all, bitems, bthr, btgt;
+    DBuf items, bitems, bthr, btgt;
     HIPCHK(items.ensure((size_t)n * sizeof(Item)));
     HIPCHK(hipMemcpyAsync(items.p, it.data(), (size_t)n * sizeof(Item), hipMemcpyHostToDevice, s));
     KParams p = base_params(ctx, 0, 1, 1);

@@ -24,6 +24,8 @@ template <int W> __device__ __forceinline__ int dir_cell(int d) {
 // Lane states of the DFS engine.
 // lane states; ST_DESC (children f-tested, descent into the first passing one
 // pending) never outlives a round
+constexpr int kQCtl = 16; // u64 stride between the two rings' counters
+
 enum : int { ST_IDLE = 0, ST_FRESH = 1, ST_POP = 2, ST_DRAINED = 3, ST_DESC = 4 };
 
 constexpr int kMaxIters = 128;      // threshold iterations kept per instance
@@ -67,7 +69,9 @@ __host__ __device__ inline int meta_prev(uint32_t m) { return (int)(m >> 24); }
 struct KParams {
     // work queue: two rings of qcap (power of two) items each, ring 0 for
     // the current iterations, ring 1 for lookahead iterations (ring r at
-    // q + r * qcap); q_ctl[3r..3r+2] = head, limit, tail of ring r
+    // q + r * qcap); q_ctl[kQCtl*r + 0..2] = head, limit, tail of ring r
+    // (each ring's counters on a 128-byte line of their own: every idle
+    // wave's grab is an atomic on them)
     Item *q;
     unsigned long long qmask;
     unsigned long long *q_ctl;
@@ -104,8 +108,9 @@ struct KParams {
                                  // 7 donated, 8 overflow exports, 9/10 big/small
                                  // budget, 12 busy at last round end,
                                  // 13 lookahead iterations started,
-                                 // 18 busy lanes on lookahead iterations,
-                                 // 19 donate_per_lane of those lanes
+                                 // 19 donate_per_lane of lanes on lookahead
+                                 // iterations, 34 how many are busy (own
+                                 // cache line: every wave adds to it)
 
     // top tree of the root split (per instance)
     const Item *frontier;        // [n_inst][capl]

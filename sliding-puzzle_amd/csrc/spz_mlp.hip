@@ -197,7 +197,7 @@ hipError_t mlp_alloc(MlpState &m) {
 }
 
 static void fr(void *p) {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
 }
 
 static void free_scratch(MlpState &m) {

@@ -97,14 +97,14 @@ __device__ __forceinline__ void put_item(const KParams &p, int r, unsigned long 
     p.q[q_slot(p, r, pos)] = it;
 }
 
-// Reserve k slots at the tail of ring r (counters q_ctl[3r..3r+2]: head,
-// limit, tail).  Items in [min(head, limit), tail) are not taken yet; a tail
+// Reserve k slots at the tail of ring r (counters q_ctl[kQCtl*r + 0..2]:
+// head, limit, tail).  Items in [min(head, limit), tail) are not taken yet; a tail
 // more than the ring's capacity past them would overwrite some, so that sets
 // the error word (ctl[3]) and the host fails the solve with SPZ_E_CAPACITY
 // rather than return counts from a corrupt queue.  head is read as it
 // stands, possibly behind, which only makes the test stricter.
 __device__ __forceinline__ unsigned long long q_reserve(const KParams &p, int r, unsigned long long k) {
-    unsigned long long *c = p.q_ctl + 3 * r;
+    unsigned long long *c = p.q_ctl + kQCtl * r;
     const unsigned long long pos = atomicAdd(&c[2], k);
     const unsigned long long h = *(volatile unsigned long long *)&c[0], lim = c[1];
     const unsigned long long live_from = h < lim ? h : lim;
@@ -196,7 +196,10 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
         __syncthreads();
     }
 
-    const unsigned long long limit = p.q_ctl[1], limit1 = p.q_ctl[4];
+    const unsigned long long limit = p.q_ctl[1], limit1 = p.q_ctl[kQCtl + 1];
+    // a ring seen empty stays empty for the round (heads only grow): the
+    // wave then skips its atomic
+    bool q0_empty = p.q_ctl[0] >= limit, q1_empty = !p.lookahead || p.q_ctl[kQCtl] >= limit1;
     const int budget = (int)p.ctl[5];
     const int mode = p.mode;
 
@@ -328,39 +331,36 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                 int n = __popcll(need);
                 int leader = __ffsll((long long)need) - 1;
                 unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(&p.q_ctl[0], (unsigned long long)n);
-                base = shfl_u64(base, leader);
-                if (st == ST_IDLE) {
-                    unsigned long long my = base + __popcll(need & lanemask_lt());
-                    if (my < limit) {
+                if (!q0_empty) {
+                    if (lane == leader) base = atomicAdd(&p.q_ctl[0], (unsigned long long)n);
+                    base = shfl_u64(base, leader);
+                    const unsigned long long my = base + __popcll(need & lanemask_lt());
+                    if (st == ST_IDLE && my < limit) {
                         start_item(p.q[q_slot(p, 0, my)]);
                         if (p.qpath)
                             path_copy(p.qpath, (uint32_t)q_slot(p, 0, my) * kPathWords, 1, p.c_ipath, (uint32_t)gl,
                                       (uint32_t)NL);
-                    } else if (!p.lookahead) {
-                        st = ST_DRAINED;
                     }
+                    q0_empty = base + (unsigned long long)n >= limit;
                 }
                 // lanes ring 0 could not serve try the lookahead ring
-                const uint64_t need1 = p.lookahead ? __ballot(st == ST_IDLE) : 0ull;
+                const uint64_t need1 = q1_empty ? 0ull : __ballot(st == ST_IDLE);
                 if (need1) {
                     const int n1 = __popcll(need1);
                     const int leader1 = __ffsll((long long)need1) - 1;
                     unsigned long long base1 = 0;
-                    if (lane == leader1) base1 = atomicAdd(&p.q_ctl[3], (unsigned long long)n1);
+                    if (lane == leader1) base1 = atomicAdd(&p.q_ctl[kQCtl], (unsigned long long)n1);
                     base1 = shfl_u64(base1, leader1);
-                    if (st == ST_IDLE) {
-                        const unsigned long long my = base1 + __popcll(need1 & lanemask_lt());
-                        if (my < limit1) {
-                            start_item(p.q[q_slot(p, 1, my)]);
-                            if (p.qpath)
-                                path_copy(p.qpath, (uint32_t)q_slot(p, 1, my) * kPathWords, 1, p.c_ipath,
-                                          (uint32_t)gl, (uint32_t)NL);
-                        } else {
-                            st = ST_DRAINED;
-                        }
+                    const unsigned long long my = base1 + __popcll(need1 & lanemask_lt());
+                    if (st == ST_IDLE && my < limit1) {
+                        start_item(p.q[q_slot(p, 1, my)]);
+                        if (p.qpath)
+                            path_copy(p.qpath, (uint32_t)q_slot(p, 1, my) * kPathWords, 1, p.c_ipath, (uint32_t)gl,
+                                      (uint32_t)NL);
                     }
+                    q1_empty = base1 + (unsigned long long)n1 >= limit1;
                 }
+                if (st == ST_IDLE) st = ST_DRAINED;
             }
         }
         if (!__ballot(st == ST_FRESH || st == ST_POP || st == ST_DESC)) break;
@@ -750,7 +750,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     uint64_t busy_mask = __ballot(busy);
     if (lane == 0 && busy_mask) atomicAdd(&p.ctl[2], (unsigned)__popcll(busy_mask));
     const uint64_t ahead_mask = p.lookahead ? __ballot(busy && ring == 1) : 0ull;
-    if (lane == 0 && ahead_mask) atomicAdd(&p.ctl[18], (unsigned)__popcll(ahead_mask));
+    if (lane == 0 && ahead_mask) atomicAdd(&p.ctl[34], (unsigned)__popcll(ahead_mask));
     if (busy) {
         p.c_st[gl] = (uint32_t)st | ((uint32_t)sp << 8);
         uint32_t b[4];
@@ -955,7 +955,8 @@ __global__ void __launch_bounds__(256) k_spec(KParams p) {
     if (threadIdx.x == 0) {
         unsigned long long qlen = 0;
         for (int r = 0; r < 2; ++r) {
-            const unsigned long long head = p.q_ctl[3 * r], lim = p.q_ctl[3 * r + 1], tail = p.q_ctl[3 * r + 2];
+            const unsigned long long *c = p.q_ctl + kQCtl * r;
+            const unsigned long long head = c[0], lim = c[1], tail = c[2];
             qlen += tail - (head < lim ? head : lim);
         }
         const unsigned long long lanes = (unsigned long long)p.n_lanes, busy = p.ctl[2];
@@ -1005,16 +1006,16 @@ __global__ void k_control(KParams p) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     unsigned long long qlen[2];
     for (int r = 0; r < 2; ++r) {
-        unsigned long long *c = p.q_ctl + 3 * r;
+        unsigned long long *c = p.q_ctl + kQCtl * r;
         unsigned long long head = c[0], limit = c[1], tail = c[2];
         if (head > limit) head = limit;
         c[0] = head;
         c[1] = tail;
         qlen[r] = tail - head;
     }
-    const unsigned busy = p.ctl[2], busy1 = p.ctl[18];
+    const unsigned busy = p.ctl[2], busy1 = p.ctl[34];
     p.ctl[2] = 0;
-    p.ctl[18] = 0;
+    p.ctl[34] = 0;
     p.ctl[12] = busy; // lanes busy at the end of the last round, for the host
     p.ctl[0] = 0;     // restart list consumed
     const unsigned long long lanes = (unsigned long long)p.n_lanes;
