@@ -63,7 +63,7 @@ def test_autotuner_keeps_results(golden):
         best, ms, tried = s.autotune(boards)
         say("best", best, ms, tried)
         assert tried >= 10 and ms > 0
-        assert best["workgroup_size"] in (64, 128, 256)
+        assert best["workgroup_size"] in (64, 128, 256) and best["lookahead"] in (0, 1)
         assert s.config()["workgroup_size"] == best["workgroup_size"]
         res = s.solve(boards)
     for rec, r in zip(fx, res):
