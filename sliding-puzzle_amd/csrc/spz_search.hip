@@ -324,7 +324,8 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     // would wait on a move's table reads before issuing the f-test's)
     __builtin_amdgcn_s_waitcnt(0);
     for (int step = 0; step < budget; ++step) {
-        // ---- idle lanes take queue items (one atomic per wave) ----
+        // ---- idle lanes take queue items: one atomic per wave and ring,
+        // ring 0 first, none on a ring the wave has seen empty ----
         {
             uint64_t need = __ballot(st == ST_IDLE);
             if (need) {
