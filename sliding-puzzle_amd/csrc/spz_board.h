@@ -314,6 +314,20 @@ __device__ __forceinline__ void do_move(Pos<W, HEUR, BD> &s, int d, const PdbCtx
     s.bc = nc;
 }
 
+// Blank move that updates the board and blank only.  Queue items of the PDB
+// search need no heuristic state: a lane starting an item recomputes all
+// eight lookups from the board (pdb_init), so the positions of donated and
+// exported items are built with these moves and no table reads.
+template <int W, int HEUR, class BD>
+__device__ __forceinline__ void move_board_only(Pos<W, HEUR, BD> &s, int d) {
+    const int nr = s.br + dir_dr(d);
+    const int nc = s.bc + dir_dc(d);
+    const int from = nr * W + nc, to = s.br * W + s.bc;
+    s.bd.slide(s.bd.tile(from), from, to);
+    s.br = nr;
+    s.bc = nc;
+}
+
 // do_move for the PDB state with the moved tile's two lookup values known
 template <int W, class BD>
 __device__ __forceinline__ void do_move_known(Pos<W, 1, BD> &s, int d, const PdbCtx &P, uint32_t vals) {

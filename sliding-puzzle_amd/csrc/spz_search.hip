@@ -611,7 +611,10 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                         pass &= pass - 1;
                         if (p.qpath) item_path(r, pos, sp, d);
                         Pos<W, HEUR> c = reg_pos(s);
-                        do_move<W, HEUR>(c, d, P);
+                        if constexpr (HEUR == 1)
+                            move_board_only(c, d); // the item's lane recomputes its lookups
+                        else
+                            do_move<W, HEUR>(c, d, P);
                         put_item<W, HEUR>(p, r, pos++, c, g + 1, d, inst, origin);
                     }
                     st = ST_POP;
@@ -696,9 +699,18 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                     }
                 }
             }
-            // replay from the item root along the taken moves
+            // replay from the item root along the taken moves (for the PDB the
+            // board alone: the lane that takes an item recomputes its lookups)
             Pos<W, HEUR> c;
-            pos_from_item<W, HEUR>(c, root, rmeta, P);
+            if constexpr (HEUR == 1) {
+                c.bd.unpack(root);
+                const int blank = meta_blank(rmeta);
+                c.br = blank / W;
+                c.bc = blank - c.br * W;
+                c.hs.h = 0;
+            } else {
+                pos_from_item<W, HEUR>(c, root, rmeta, P);
+            }
             int cg = meta_g(rmeta);
             int emitted = 0;
             for (int k = 0; k < sp && emitted < want; ++k) {
@@ -709,13 +721,19 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                     int d2 = __builtin_ctz(R);
                     R &= R - 1;
                     Pos<W, HEUR> x = c;
-                    do_move<W, HEUR>(x, d2, P);
+                    if constexpr (HEUR == 1)
+                        move_board_only(x, d2);
+                    else
+                        do_move<W, HEUR>(x, d2, P);
                     put_item<W, HEUR>(p, ring, pos++, x, cg + 1, d2, inst, origin);
                     ++emitted;
                 }
                 stk[k * sstride] = SE::make(d, R, SE::vals(e));
                 if (emitted < want) {
-                    do_move<W, HEUR>(c, d, P);
+                    if constexpr (HEUR == 1)
+                        move_board_only(c, d);
+                    else
+                        do_move<W, HEUR>(c, d, P);
                     ++cg;
                 }
             }
