@@ -70,9 +70,11 @@ struct spz_ctx {
     int n_cu = 0;
     int n_lanes = 0;
     int wg = 256;
+    double wall_ticks_per_us = 0; // device wall clock (wall_clock64) rate
     std::string err;
     double stats[24] = {0};
     DBuf prof;             // lane-state counters (env SPZ_PROFILE=1)
+    DBuf wtime;            // per-wave clocks of one round (env SPZ_WAVE_TIMES)
     double timeout_s = 0;
 
     PdbGeo geo_h;
@@ -178,6 +180,13 @@ void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic) {
 // 2048 is level with it).
 static int default_round_budget(int) { return 1024; }
 
+// Round length by time when the configuration leaves it open: a fixed step
+// budget lets waves that step faster (fewer lanes on divergent paths) finish
+// early and wait for the slowest -- in the 24-puzzle PDB search waves end
+// 4.75 ms (10th percentile) to 5.5 ms (last) into a 1024-step round.  The
+// time limit ends all waves together.  Manhattan rounds keep the step budget.
+static int default_round_time_us(int heur) { return heur == SPZ_H_PDB666 ? 4800 : -1; }
+
 // Lanes per compute unit of the 24-puzzle searches: more resident lanes than
 // this lose more to L2 sharing and finer work splitting than they gain in
 // latency hiding (measured on MI355X: 768 beat 1024 and 512 for both
@@ -262,6 +271,13 @@ int spz_create(spz_ctx **out, const spz_config *cfg) {
             break;
         }
         ctx->n_cu = prop.multiProcessorCount;
+        {
+            int khz = 0;
+            if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, cfg->device) == hipSuccess && khz > 0)
+                ctx->wall_ticks_per_us = khz / 1000.0;
+            else
+                (void)hipGetLastError();
+        }
         e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
             rc = fail(ctx, SPZ_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
@@ -395,7 +411,8 @@ void spz_destroy(spz_ctx *ctx) {
                    &ctx->pred,        &ctx->qpath,          &ctx->c_ipath,        &ctx->fpath,
                    &ctx->lpath[0],    &ctx->lpath[1],       &ctx->sol,            &ctx->sol_len,
                    &ctx->itn,         &ctx->drained,        &ctx->nlive,          &ctx->Lbase,
-                   &ctx->listed,      &ctx->want_next,      &ctx->dead};
+                   &ctx->listed,      &ctx->want_next,      &ctx->dead,           &ctx->prof,
+                   &ctx->wtime};
     for (DBuf *b : all) b->release();
     if (ctx->mlp_alloced) mlp_free(ctx->mlp);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -544,7 +561,7 @@ static KParams base_params(spz_ctx *ctx, int n, int max_iters, int capl) {
     p.capl = capl;
     p.n_lanes = ctx->n_lanes;
     p.maxd = ctx->cfg.lds_stack_depth;
-    p.prof = ctx->prof.as<unsigned long long>();
+    p.prof = nullptr; // a solve turns the counters on (env SPZ_PROFILE)
     p.c_st = ctx->c_st.as<uint32_t>();
     p.c_rows = ctx->c_rows.as<uint32_t>();
     p.c_root = ctx->c_root.as<uint32_t>();
@@ -735,6 +752,13 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         HIPCHK(ctx->prof.ensure(64));
         HIPCHK(hipMemsetAsync(ctx->prof.p, 0, 64, s));
         p.prof = ctx->prof.as<unsigned long long>();
+        // with SPZ_WAVE_TIMES=r: every wave's start and end clock in round r
+        if (getenv("SPZ_WAVE_TIMES")) {
+            HIPCHK(ctx->wtime.ensure(lanes / 64 * 16));
+            HIPCHK(hipMemsetAsync(ctx->wtime.p, 0, lanes / 64 * 16, s));
+            p.wtime = ctx->wtime.as<unsigned long long>();
+            p.wtime_round = (unsigned)atoi(getenv("SPZ_WAVE_TIMES"));
+        }
     }
     HIPCHK(hipMemsetAsync(ctx->load.p, 0, front_items * 8, s));
     HIPCHK(hipMemsetAsync(ctx->nsamp.p, 0, 64, s));
@@ -773,7 +797,12 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         ctl[0] = (unsigned)n; // restart list length
         ctl[1] = (unsigned)n; // active instances
         int big = ctx->cfg.round_budget > 0 ? ctx->cfg.round_budget : default_round_budget(heur);
-        ctl[9] = (unsigned)big;
+        // rounds that end by time: the step budget becomes a cap above it
+        const int rt = heur != SPZ_H_PDB666 ? -1
+                       : ctx->cfg.round_time_us == 0 ? default_round_time_us(heur) : ctx->cfg.round_time_us;
+        p.round_ticks = (rt > 0 && ctx->wall_ticks_per_us > 0)
+                            ? (unsigned long long)(rt * ctx->wall_ticks_per_us) : 0ull;
+        ctl[9] = (unsigned)(p.round_ticks ? 2 * big : big);
         ctl[10] = (unsigned)std::min(big, 32);
         ctl[5] = ctl[10];
         HIPCHK(hipMemcpyAsync(ctx->ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, s));
@@ -939,6 +968,25 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         unsigned long long pf[7];
         HIPCHK(hipMemcpy(pf, p.prof, sizeof pf, hipMemcpyDeviceToHost));
         for (int k = 0; k < 7; ++k) ctx->stats[16 + k] = (double)pf[k];
+    }
+    if (p.wtime) {
+        // end times of the recorded round's waves after its first start, on
+        // stderr: the spread is the round's tail
+        const size_t nw = lanes / 64;
+        std::vector<unsigned long long> wt(2 * nw);
+        HIPCHK(hipMemcpy(wt.data(), p.wtime, wt.size() * 8, hipMemcpyDeviceToHost));
+        std::vector<double> ends;
+        unsigned long long t0 = ~0ull;
+        for (size_t k = 0; k < nw; ++k)
+            if (wt[2 * k + 1]) t0 = std::min(t0, wt[2 * k]);
+        for (size_t k = 0; k < nw; ++k)
+            if (wt[2 * k + 1]) ends.push_back((double)(wt[2 * k + 1] - t0));
+        std::sort(ends.begin(), ends.end());
+        if (!ends.empty()) {
+            auto q = [&](double f) { return ends[(size_t)(f * (ends.size() - 1))]; };
+            fprintf(stderr, "spz wave end ticks in round %u (%zu waves): p10 %.0f p50 %.0f p90 %.0f p99 %.0f max %.0f\n",
+                    p.wtime_round, ends.size(), q(0.1), q(0.5), q(0.9), q(0.99), ends.back());
+        }
     }
     ctx->stats[12] = all_nodes;
     return SPZ_OK;

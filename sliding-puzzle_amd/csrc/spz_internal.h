@@ -132,6 +132,9 @@ struct KParams {
     int n_lanes;
     int maxd;          // LDS stack levels per lane in use
     unsigned long long *prof; // null, or [4]: lane-steps idle, expanding, popping only, wave-steps
+    unsigned long long round_ticks; // big rounds end this many wall-clock ticks after a wave's start (0: off)
+    unsigned long long *wtime; // null, or [2][waves]: start and end clock of each wave in round wtime_round
+    unsigned wtime_round;
     uint32_t *c_st;    // state | sp<<8
     uint32_t *c_rows;  // [5][n_lanes]
     uint32_t *c_root;  // [4][n_lanes] packed item root

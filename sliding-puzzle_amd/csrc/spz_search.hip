@@ -174,6 +174,8 @@ __device__ __forceinline__ Pos<W, HEUR> reg_pos(const Pos<W, HEUR, BD> &s) {
 // their LDS (stack and boards) at 3 waves per SIMD.
 template <int W, int HEUR>
 __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, const PdbGeo *geo_g) {
+    // every wave of the grid starts within microseconds of the others
+    const long long t_start = (p.wtime || (HEUR == 1 && p.round_ticks)) ? wall_clock64() : 0;
     const int MAXD = p.maxd;
     extern __shared__ uint8_t lds[];
     __shared__ PdbGeo geo_s;
@@ -324,6 +326,13 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     // would wait on a move's table reads before issuing the f-test's)
     __builtin_amdgcn_s_waitcnt(0);
     for (int step = 0; step < budget; ++step) {
+        // a round with a time limit (PDB search) ends between two steps once
+        // it is up (the clock is a scalar read, wave-uniform)
+        if constexpr (HEUR == 1) {
+            if (p.round_ticks && (step & 3) == 3 &&
+                (unsigned long long)(wall_clock64() - t_start) >= p.round_ticks)
+                break;
+        }
         // ---- idle lanes take queue items: one atomic per wave and ring,
         // ring 0 first, none on a ring the wave has seen empty ----
         {
@@ -765,6 +774,11 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
             atomicAdd(&p.prof[5], (unsigned long long)pf_rd1);
             atomicAdd(&p.prof[6], (unsigned long long)pf_rd2);
         }
+    }
+    if (p.wtime && lane == 0 && p.ctl[6] == p.wtime_round) {
+        const int wg = gl >> 6; // global wave index
+        p.wtime[2 * wg] = (unsigned long long)t_start;
+        p.wtime[2 * wg + 1] = (unsigned long long)wall_clock64();
     }
     uint64_t busy_mask = __ballot(busy);
     if (lane == 0 && busy_mask) atomicAdd(&p.ctl[2], (unsigned)__popcll(busy_mask));

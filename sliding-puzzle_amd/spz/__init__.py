@@ -48,7 +48,7 @@ class Config(ctypes.Structure):
                 ("time_kernels", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("log_samples", ctypes.c_int32),
                 ("queue_items", ctypes.c_int64), ("record_paths", ctypes.c_int32),
-                ("lookahead", ctypes.c_int32)]
+                ("lookahead", ctypes.c_int32), ("round_time_us", ctypes.c_int32)]
 
 
 class _Result(ctypes.Structure):
@@ -185,7 +185,8 @@ class Solver:
                  workgroup_size: int = 256, lanes: int = 0, round_budget: int = 0,
                  lb_policy: int = LB_UNIFORM, device: int = 0, queue_items: int = 0,
                  time_kernels: bool = False, lds_stack_depth: int = 0, log_samples: int = 0,
-                 use_graph: bool = True, record_paths: bool = False, lookahead: bool = True):
+                 use_graph: bool = True, record_paths: bool = False, lookahead: bool = True,
+                 round_time_us: int = 0):
         L = lib()
         cfg = Config()
         L.spz_default_config(ctypes.byref(cfg), width, heuristic)
@@ -204,6 +205,7 @@ class Solver:
         cfg.use_graph = 1 if use_graph else 0
         cfg.record_paths = 1 if record_paths else 0
         cfg.lookahead = 1 if lookahead else 0
+        cfg.round_time_us = round_time_us
         self.cfg = cfg
         self.width = width
         self._ctx = ctypes.c_void_p()
