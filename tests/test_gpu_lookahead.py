@@ -98,3 +98,20 @@ def test_lookahead_off_under_load_policy():
     with spz.Solver(width=4, lb_policy=spz.LB_PREVLOAD) as s:
         s.solve(boards)
         assert s.stats()["lookahead_starts"] == 0
+
+
+def test_round_time_limit_keeps_results(ref_sets, ref_results):
+    """PDB rounds that end by time (default), by step budget (off) and by a
+    short time limit give the same lengths and per-iteration counts."""
+    rows = ref_results["pdb_expand_nodes"]["rows"]
+    idx = [i for i, r in enumerate(rows) if r["nodes"] < 30_000_000][:6]
+    boards = [ref_sets["yama24_50_hard_new"][i] for i in idx]
+    boards += [O.walk(5, 120, 5_600_000 + i) for i in range(6)]
+    kw = dict(width=5, heuristic=spz.H_PDB666, frontier_target=1024)
+    base, _ = results(boards, **kw)
+    off, _ = results(boards, round_time_us=-1, **kw)
+    short, st = results(boards, round_time_us=300, **kw)
+    say("rounds with a 300 us limit", st["rounds"])
+    assert base == off == short
+    for k, i in enumerate(idx):
+        assert base[k][0] == rows[i]["len"]
