@@ -32,7 +32,9 @@ static void usage() {
                  "  --lb uniform|prevload  dispatch order of frontier nodes\n"
                  "  --frontier N --wg N --lds-depth N --budget N --max-iters N\n"
                  "  --autotune             tune the launch configuration on the files first\n"
-                 "  --no-lookahead         run each instance's iterations strictly one at a time\n");
+                 "  --no-lookahead         run each instance's iterations strictly one at a time\n"
+                 "  --round-time US        PDB search rounds end after US microseconds (-1: after\n"
+                 "                         the step budget; default 4800)\n");
 }
 
 static bool read_board(const std::string &path, std::vector<uint8_t> &b, int &width) {
@@ -52,7 +54,7 @@ static bool read_board(const std::string &path, std::vector<uint8_t> &b, int &wi
 int main(int argc, char **argv) {
     spz_config cfg;
     int heur = SPZ_H_MANHATTAN, mode = SPZ_MODE_ALL, lb = SPZ_LB_UNIFORM;
-    int frontier = 2048, wg = 256, depth = 0, budget = 0, max_iters = 64;
+    int frontier = 2048, wg = 256, depth = 0, budget = 0, max_iters = 64, round_us = 0;
     bool batch = false, iters = false, autotune = false, path = false, lookahead = true;
     std::string csv, cache, tabdir;
     std::vector<std::string> files;
@@ -93,6 +95,8 @@ int main(int argc, char **argv) {
             depth = std::atoi(next());
         } else if (a == "--budget") {
             budget = std::atoi(next());
+        } else if (a == "--round-time") {
+            round_us = std::atoi(next());
         } else if (a == "--max-iters") {
             max_iters = std::atoi(next());
         } else if (a == "--no-lookahead") {
@@ -130,6 +134,7 @@ int main(int argc, char **argv) {
     cfg.lb_policy = lb;
     cfg.record_paths = path ? 1 : 0;
     cfg.lookahead = lookahead ? 1 : 0;
+    cfg.round_time_us = round_us;
     spz_ctx *ctx = nullptr;
     int rc = spz_create(&ctx, &cfg);
     if (rc != SPZ_OK) {
