@@ -99,12 +99,12 @@ typedef struct spz_config {
                                 SPZ_LB_PREVLOAD and sample logging.  0: one
                                 iteration at a time, as in the reference
                                 (src/24puzzle.cc:146-152) */
-    int32_t round_time_us;   /* PDB search: rounds end by time, every wave
-                                stops this many microseconds after the round's
-                                start (at most 2x round_budget steps), so waves
-                                that step faster do not wait for the slowest;
-                                0 = auto (4800), < 0 = off (rounds end after
-                                round_budget steps, as Manhattan rounds do) */
+    int32_t round_time_us;   /* a search round also ends this many
+                                microseconds after its start (whichever comes
+                                first with round_budget steps), so that waves
+                                that step faster under load do not wait for
+                                the slowest; 0 = auto (4500 for the PDB search,
+                                2500 for Manhattan), < 0 = off */
 } spz_config;
 
 typedef struct spz_result {

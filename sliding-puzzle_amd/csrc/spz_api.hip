@@ -180,12 +180,14 @@ void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic) {
 // 2048 is level with it).
 static int default_round_budget(int) { return 1024; }
 
-// Round length by time when the configuration leaves it open: a fixed step
-// budget lets waves that step faster (fewer lanes on divergent paths) finish
-// early and wait for the slowest -- in the 24-puzzle PDB search waves end
-// 4.75 ms (10th percentile) to 5.5 ms (last) into a 1024-step round.  The
-// time limit ends all waves together.  Manhattan rounds keep the step budget.
-static int default_round_time_us(int heur) { return heur == SPZ_H_PDB666 ? 4800 : -1; }
+// Time limit of a round when the configuration leaves it open.  With a step
+// budget alone, waves that step faster (fewer lanes on divergent paths) end
+// early and wait for the slowest -- in the 24-puzzle PDB search under load,
+// waves end 4.75 ms (10th percentile) to 5.5 ms (last) into a 1024-step
+// round.  The limit cuts that tail; lightly loaded rounds, which step faster,
+// still end at their step budget (measured on MI355X: PDB 4500 us beat 3200,
+// 4000, 4800 and 5600; Manhattan 2500 us beat 1200, 2000 and 3000).
+static int default_round_time_us(int heur) { return heur == SPZ_H_PDB666 ? 4500 : 2500; }
 
 // Lanes per compute unit of the 24-puzzle searches: more resident lanes than
 // this lose more to L2 sharing and finer work splitting than they gain in
@@ -797,12 +799,12 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         ctl[0] = (unsigned)n; // restart list length
         ctl[1] = (unsigned)n; // active instances
         int big = ctx->cfg.round_budget > 0 ? ctx->cfg.round_budget : default_round_budget(heur);
-        // rounds that end by time: the step budget becomes a cap above it
-        const int rt = heur != SPZ_H_PDB666 ? -1
-                       : ctx->cfg.round_time_us == 0 ? default_round_time_us(heur) : ctx->cfg.round_time_us;
+        // a round ends after its step budget or its time limit, whichever
+        // comes first
+        const int rt = ctx->cfg.round_time_us == 0 ? default_round_time_us(heur) : ctx->cfg.round_time_us;
         p.round_ticks = (rt > 0 && ctx->wall_ticks_per_us > 0)
                             ? (unsigned long long)(rt * ctx->wall_ticks_per_us) : 0ull;
-        ctl[9] = (unsigned)(p.round_ticks ? 2 * big : big);
+        ctl[9] = (unsigned)big;
         ctl[10] = (unsigned)std::min(big, 32);
         ctl[5] = ctl[10];
         HIPCHK(hipMemcpyAsync(ctx->ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, s));

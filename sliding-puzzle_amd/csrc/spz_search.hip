@@ -175,7 +175,7 @@ __device__ __forceinline__ Pos<W, HEUR> reg_pos(const Pos<W, HEUR, BD> &s) {
 template <int W, int HEUR>
 __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, const PdbGeo *geo_g) {
     // every wave of the grid starts within microseconds of the others
-    const long long t_start = (p.wtime || (HEUR == 1 && p.round_ticks)) ? wall_clock64() : 0;
+    const long long t_start = (p.wtime || p.round_ticks) ? wall_clock64() : 0;
     const int MAXD = p.maxd;
     extern __shared__ uint8_t lds[];
     __shared__ PdbGeo geo_s;
@@ -328,11 +328,8 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     for (int step = 0; step < budget; ++step) {
         // a round with a time limit (PDB search) ends between two steps once
         // it is up (the clock is a scalar read, wave-uniform)
-        if constexpr (HEUR == 1) {
-            if (p.round_ticks && (step & 3) == 3 &&
-                (unsigned long long)(wall_clock64() - t_start) >= p.round_ticks)
-                break;
-        }
+        if (p.round_ticks && (step & 3) == 3 && (unsigned long long)(wall_clock64() - t_start) >= p.round_ticks)
+            break;
         // ---- idle lanes take queue items: one atomic per wave and ring,
         // ring 0 first, none on a ring the wave has seen empty ----
         {

@@ -33,8 +33,8 @@ static void usage() {
                  "  --frontier N --wg N --lds-depth N --budget N --max-iters N\n"
                  "  --autotune             tune the launch configuration on the files first\n"
                  "  --no-lookahead         run each instance's iterations strictly one at a time\n"
-                 "  --round-time US        PDB search rounds end after US microseconds (-1: after\n"
-                 "                         the step budget; default 4800)\n");
+                 "  --round-time US        a search round also ends after US microseconds (-1: off;\n"
+                 "                         default 4500 for the PDB search, 2500 for Manhattan)\n");
 }
 
 static bool read_board(const std::string &path, std::vector<uint8_t> &b, int &width) {
