@@ -326,8 +326,8 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     // would wait on a move's table reads before issuing the f-test's)
     __builtin_amdgcn_s_waitcnt(0);
     for (int step = 0; step < budget; ++step) {
-        // a round with a time limit (PDB search) ends between two steps once
-        // it is up (the clock is a scalar read, wave-uniform)
+        // a round with a time limit ends between two steps once it is up
+        // (the clock is a scalar read, wave-uniform)
         if (p.round_ticks && (step & 3) == 3 && (unsigned long long)(wall_clock64() - t_start) >= p.round_ticks)
             break;
         // ---- idle lanes take queue items: one atomic per wave and ring,
