@@ -132,8 +132,8 @@ void spz_get_config(spz_ctx *ctx, spz_config *cfg);
 /* Autotuner (no reference counterpart; the reference's shared-memory sweep
  * was done by hand, main.tex:742): coordinate descent over workgroup size
  * 64/128/256, root-split frontier target (= split depth), LDS stack depth,
- * round budget and lookahead on/off, timing spz_solve_batch on the sample
- * boards.  Every
+ * round budget, lookahead on/off and the round time limit on/off, timing
+ * spz_solve_batch on the sample boards.  Every
  * candidate must reproduce the first one's lengths and per-iteration
  * counts (SPZ_E_STALL otherwise).  The context keeps the fastest
  * configuration, also returned in *best. */

@@ -321,9 +321,9 @@ void spz_get_config(spz_ctx *ctx, spz_config *cfg) {
 }
 
 // Coordinate descent over workgroup size, root-split target, LDS stack depth,
-// round budget and lookahead on/off on a sample batch; every candidate must
-// give the same lengths and counts.  The context is left configured with the
-// winner.
+// round budget, lookahead on/off and the round time limit on/off on a sample
+// batch; every candidate must give the same lengths and counts.  The context
+// is left configured with the winner.
 int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_iters, int32_t passes,
                  spz_config *best, double *best_ms, int32_t *tried) {
     if (!ctx || !boards || n <= 0 || !best) return SPZ_E_ARG;
@@ -364,8 +364,8 @@ int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_ite
     if (rc) return rc;
     count = 2;
     for (int pass = 0; pass < passes; ++pass) {
-        for (int dim = 0; dim < 5; ++dim) {
-            int nv = dim == 0 ? 3 : dim == 1 ? 5 : dim == 4 ? 2 : 4;
+        for (int dim = 0; dim < 6; ++dim) {
+            int nv = dim == 0 ? 3 : dim == 1 ? 5 : dim >= 4 ? 2 : 4;
             spz_config winner = bc;
             for (int v = 0; v < nv; ++v) {
                 spz_config c = bc;
@@ -374,6 +374,7 @@ int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_ite
                 if (dim == 2) c.lds_stack_depth = deps[v];
                 if (dim == 3) c.round_budget = buds[v];
                 if (dim == 4) c.lookahead = v;
+                if (dim == 5) c.round_time_us = v ? -1 : 0; // time limit (auto) or step budget only
                 if (memcmp(&c, &bc, sizeof c) == 0) continue;
                 c.lanes = 0;
                 double ms = 0;
