@@ -105,6 +105,15 @@ typedef struct spz_config {
                                 that step faster under load do not wait for
                                 the slowest; 0 = auto (4500 for the PDB search,
                                 2500 for Manhattan), < 0 = off */
+    int32_t dedup;           /* 1: duplicate pruning in the root split: a
+                                top-tree node whose board an earlier kept node
+                                of its instance holds with a path-max f no
+                                larger is dropped before it is expanded (the
+                                closed-list root distribution of
+                                src_horie/idas_smem.cu:1057-1149).  Optimal
+                                lengths are unchanged; node counts become
+                                smaller than the reference's, so parity runs
+                                leave it off (default 0) */
 } spz_config;
 
 typedef struct spz_result {

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 namespace spz {
@@ -530,6 +531,31 @@ int spz_export_pdb_tab(spz_ctx *ctx, const char *p0, const char *p1) {
 }
 
 // -------------------------------------------------------------- solve ---
+// Duplicate pruning in the root split (config dedup).  Before a level is
+// expanded, a node is dropped when an earlier kept node of its instance's top
+// tree holds the same board with a path-max f no larger than its own: that
+// node reaches everything the dropped one would within any threshold (its g
+// is no larger, the heuristic of a board is the same), so optimal lengths
+// stay; the dropped node's subtree is no longer counted.  Host side, between
+// split levels: it changes the set-up only, never the search kernels.
+struct BoardKey {
+    uint32_t w[4];
+    bool operator==(const BoardKey &o) const {
+        return w[0] == o.w[0] && w[1] == o.w[1] && w[2] == o.w[2] && w[3] == o.w[3];
+    }
+};
+struct BoardKeyHash {
+    size_t operator()(const BoardKey &k) const {
+        uint64_t h = 0x9E3779B97F4A7C15ull;
+        for (int i = 0; i < 4; ++i) h = (h ^ k.w[i]) * 0xBF58476D1CE4E5B9ull;
+        return (size_t)(h ^ (h >> 31));
+    }
+};
+using SeenBoards = std::unordered_map<BoardKey, int, BoardKeyHash>;
+
+static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur, bool paths,
+                              std::vector<SeenBoards> &seen, hipStream_t s);
+
 static KParams base_params(spz_ctx *ctx, int n, int max_iters, int capl) {
     KParams p;
     memset(&p, 0, sizeof p);
@@ -738,7 +764,9 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     const int max_depth = ctx->cfg.max_split_depth;
     auto t_split = std::chrono::steady_clock::now();
     int cur = 0;
+    std::vector<SeenBoards> seen(ctx->cfg.dedup ? N : 0);
     for (int depth = 0; depth <= max_depth; ++depth) {
+        if (ctx->cfg.dedup && depth > 0) HIPCHK(dedup_level(ctx, n, capl, w, cur, paths, seen, s));
         HIPCHK(hipMemsetAsync(ctx->nlvl[cur ^ 1].p, 0, N * 4, s));
         HIPCHK(launch_split_level(w, heur, p, ctx->geo_d.as<PdbGeo>(), ctx->lvl[cur].as<Item>(),
                                   ctx->nlvl[cur].as<unsigned>(), ctx->lvl[cur ^ 1].as<Item>(),
@@ -993,6 +1021,45 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     }
     ctx->stats[12] = all_nodes;
     return SPZ_OK;
+}
+
+static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur, bool paths,
+                              std::vector<SeenBoards> &seen, hipStream_t s) {
+    const size_t N = (size_t)n, C = (size_t)capl;
+    std::vector<unsigned> nl(N);
+    std::vector<Item> items(N * C);
+    std::vector<unsigned long long> lp(paths ? N * C : 0);
+    hipError_t e = hipMemcpyAsync(nl.data(), ctx->nlvl[cur].p, N * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(items.data(), ctx->lvl[cur].p, N * C * sizeof(Item), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && paths) e = hipMemcpyAsync(lp.data(), ctx->lpath[cur].p, N * C * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    for (size_t i = 0; i < N; ++i) {
+        const size_t cnt = std::min<size_t>(nl[i], C);
+        size_t out = 0;
+        for (size_t j = 0; j < cnt; ++j) {
+            const Item &it = items[i * C + j];
+            BoardKey k{{it.b[0], it.b[1], width == 4 ? 0u : it.b[2], width == 4 ? 0u : it.b[3]}};
+            const int F = std::max((int)it.x0, meta_g(it.meta) + meta_h(it.meta));
+            auto f = seen[i].find(k);
+            if (f != seen[i].end() && f->second <= F) continue; // a kept copy covers it
+            if (f == seen[i].end())
+                seen[i].emplace(k, F);
+            else
+                f->second = F;
+            if (out != j) {
+                items[i * C + out] = it;
+                if (paths) lp[i * C + out] = lp[i * C + j];
+            }
+            ++out;
+        }
+        nl[i] = (unsigned)out;
+    }
+    e = hipMemcpyAsync(ctx->nlvl[cur].p, nl.data(), N * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(ctx->lvl[cur].p, items.data(), N * C * sizeof(Item), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && paths) e = hipMemcpyAsync(ctx->lpath[cur].p, lp.data(), N * C * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
 }
 
 int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens) {

@@ -33,6 +33,8 @@ static void usage() {
                  "  --frontier N --wg N --lds-depth N --budget N --max-iters N\n"
                  "  --autotune             tune the launch configuration on the files first\n"
                  "  --no-lookahead         run each instance's iterations strictly one at a time\n"
+                 "  --dedup                drop duplicate boards from the root split (fewer nodes;\n"
+                 "                         not the reference's counts)\n"
                  "  --round-time US        a search round also ends after US microseconds (-1: off;\n"
                  "                         default 4500 for the PDB search, 2500 for Manhattan)\n");
 }
@@ -55,7 +57,7 @@ int main(int argc, char **argv) {
     spz_config cfg;
     int heur = SPZ_H_MANHATTAN, mode = SPZ_MODE_ALL, lb = SPZ_LB_UNIFORM;
     int frontier = 2048, wg = 256, depth = 0, budget = 0, max_iters = 64, round_us = 0;
-    bool batch = false, iters = false, autotune = false, path = false, lookahead = true;
+    bool batch = false, iters = false, autotune = false, path = false, lookahead = true, dedup = false;
     std::string csv, cache, tabdir;
     std::vector<std::string> files;
     for (int i = 1; i < argc; ++i) {
@@ -99,6 +101,8 @@ int main(int argc, char **argv) {
             round_us = std::atoi(next());
         } else if (a == "--max-iters") {
             max_iters = std::atoi(next());
+        } else if (a == "--dedup") {
+            dedup = true;
         } else if (a == "--no-lookahead") {
             lookahead = false;
         } else if (a == "--autotune") {
@@ -135,6 +139,7 @@ int main(int argc, char **argv) {
     cfg.record_paths = path ? 1 : 0;
     cfg.lookahead = lookahead ? 1 : 0;
     cfg.round_time_us = round_us;
+    cfg.dedup = dedup ? 1 : 0;
     spz_ctx *ctx = nullptr;
     int rc = spz_create(&ctx, &cfg);
     if (rc != SPZ_OK) {

@@ -48,7 +48,8 @@ class Config(ctypes.Structure):
                 ("time_kernels", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("log_samples", ctypes.c_int32),
                 ("queue_items", ctypes.c_int64), ("record_paths", ctypes.c_int32),
-                ("lookahead", ctypes.c_int32), ("round_time_us", ctypes.c_int32)]
+                ("lookahead", ctypes.c_int32), ("round_time_us", ctypes.c_int32),
+                ("dedup", ctypes.c_int32)]
 
 
 class _Result(ctypes.Structure):
@@ -186,7 +187,7 @@ class Solver:
                  lb_policy: int = LB_UNIFORM, device: int = 0, queue_items: int = 0,
                  time_kernels: bool = False, lds_stack_depth: int = 0, log_samples: int = 0,
                  use_graph: bool = True, record_paths: bool = False, lookahead: bool = True,
-                 round_time_us: int = 0):
+                 round_time_us: int = 0, dedup: bool = False):
         L = lib()
         cfg = Config()
         L.spz_default_config(ctypes.byref(cfg), width, heuristic)
@@ -206,6 +207,7 @@ class Solver:
         cfg.record_paths = 1 if record_paths else 0
         cfg.lookahead = 1 if lookahead else 0
         cfg.round_time_us = round_time_us
+        cfg.dedup = 1 if dedup else 0
         self.cfg = cfg
         self.width = width
         self._ctx = ctypes.c_void_p()

@@ -82,6 +82,7 @@ def test_default_config_values():
     assert cfg.workgroup_size == 256 and cfg.time_kernels == 0
     assert cfg.lookahead == 1 and cfg.record_paths == 0 and cfg.use_graph == 1
     assert cfg.round_time_us == 0  # auto: 4500 us for the PDB search, 2500 for Manhattan
+    assert cfg.dedup == 0  # parity: the reference's counts
 
 
 def test_config_layout_matches_c(tmp_path):
