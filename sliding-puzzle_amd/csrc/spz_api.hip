@@ -1034,21 +1034,45 @@ static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur,
     if (e == hipSuccess && paths) e = hipMemcpyAsync(lp.data(), ctx->lpath[cur].p, N * C * 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
+    // within a level the copy kept is the one with the smallest (path-max f,
+    // previous move): the level's order comes from the split kernel's atomics,
+    // and copies equal in both are the same node, so the result does not
+    // depend on that order
+    std::unordered_map<BoardKey, size_t, BoardKeyHash> best;
+    std::vector<char> keep;
     for (size_t i = 0; i < N; ++i) {
         const size_t cnt = std::min<size_t>(nl[i], C);
-        size_t out = 0;
+        auto key = [&](const Item &it) {
+            return BoardKey{{it.b[0], it.b[1], width == 4 ? 0u : it.b[2], width == 4 ? 0u : it.b[3]}};
+        };
+        auto F = [](const Item &it) { return std::max((int)it.x0, meta_g(it.meta) + meta_h(it.meta)); };
+        best.clear();
+        keep.assign(cnt, 0);
         for (size_t j = 0; j < cnt; ++j) {
             const Item &it = items[i * C + j];
-            BoardKey k{{it.b[0], it.b[1], width == 4 ? 0u : it.b[2], width == 4 ? 0u : it.b[3]}};
-            const int F = std::max((int)it.x0, meta_g(it.meta) + meta_h(it.meta));
-            auto f = seen[i].find(k);
-            if (f != seen[i].end() && f->second <= F) continue; // a kept copy covers it
-            if (f == seen[i].end())
-                seen[i].emplace(k, F);
+            auto b = best.emplace(key(it), j);
+            if (!b.second) {
+                const Item &o = items[i * C + b.first->second];
+                if (std::make_pair(F(it), meta_prev(it.meta)) < std::make_pair(F(o), meta_prev(o.meta)))
+                    b.first->second = j;
+            }
+        }
+        for (auto &kv : best) {
+            const Item &it = items[i * C + kv.second];
+            const int f = F(it);
+            auto sf = seen[i].find(kv.first);
+            if (sf != seen[i].end() && sf->second <= f) continue; // a shallower kept copy covers it
+            if (sf == seen[i].end())
+                seen[i].emplace(kv.first, f);
             else
-                f->second = F;
+                sf->second = f;
+            keep[kv.second] = 1;
+        }
+        size_t out = 0;
+        for (size_t j = 0; j < cnt; ++j) {
+            if (!keep[j]) continue;
             if (out != j) {
-                items[i * C + out] = it;
+                items[i * C + out] = items[i * C + j];
                 if (paths) lp[i * C + out] = lp[i * C + j];
             }
             ++out;

@@ -79,3 +79,12 @@ def test_dedup_paths_stay_optimal(golden):
             b[z], b[t] = b[t], 0
             z = t
         assert b == list(range(16))
+
+
+def test_dedup_is_deterministic():
+    """Which duplicate a level keeps does not depend on the split kernel's
+    output order: repeated solves give the same counts."""
+    boards = [O.walk(4, 70, 5_800_000 + i) for i in range(12)]
+    a, _ = solve(boards, width=4, frontier_target=4096, dedup=True)
+    b, _ = solve(boards, width=4, frontier_target=4096, dedup=True)
+    assert [(r.opt_len, r.counts) for r in a] == [(r.opt_len, r.counts) for r in b]
