@@ -11,7 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <string>
-#include <unordered_map>
+#include <thread>
 #include <vector>
 
 namespace spz {
@@ -540,18 +540,18 @@ int spz_export_pdb_tab(spz_ctx *ctx, const char *p0, const char *p1) {
 // split levels: it changes the set-up only, never the search kernels.
 struct BoardKey {
     uint32_t w[4];
+    bool operator<(const BoardKey &o) const {
+        for (int i = 0; i < 4; ++i)
+            if (w[i] != o.w[i]) return w[i] < o.w[i];
+        return false;
+    }
     bool operator==(const BoardKey &o) const {
         return w[0] == o.w[0] && w[1] == o.w[1] && w[2] == o.w[2] && w[3] == o.w[3];
     }
 };
-struct BoardKeyHash {
-    size_t operator()(const BoardKey &k) const {
-        uint64_t h = 0x9E3779B97F4A7C15ull;
-        for (int i = 0; i < 4; ++i) h = (h ^ k.w[i]) * 0xBF58476D1CE4E5B9ull;
-        return (size_t)(h ^ (h >> 31));
-    }
-};
-using SeenBoards = std::unordered_map<BoardKey, int, BoardKeyHash>;
+// an instance's kept top-tree boards with the smallest path-max f kept for
+// each, sorted by board
+using SeenBoards = std::vector<std::pair<BoardKey, int>>;
 
 static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur, bool paths,
                               std::vector<SeenBoards> &seen, hipStream_t s);
@@ -1037,37 +1037,50 @@ static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur,
     // within a level the copy kept is the one with the smallest (path-max f,
     // previous move): the level's order comes from the split kernel's atomics,
     // and copies equal in both are the same node, so the result does not
-    // depend on that order
-    std::unordered_map<BoardKey, size_t, BoardKeyHash> best;
-    std::vector<char> keep;
-    for (size_t i = 0; i < N; ++i) {
+    // depend on that order.  Per instance: sort the level by (board, f,
+    // previous move), take each board's first entry, merge-join it with the
+    // boards kept at shallower levels.  Instances are independent and split
+    // over host threads.
+    struct Entry {
+        BoardKey k;
+        int f, prev;
+        uint32_t j;
+        bool operator<(const Entry &o) const {
+            if (!(k == o.k)) return k < o.k;
+            if (f != o.f) return f < o.f;
+            return prev < o.prev;
+        }
+    };
+    auto one = [&](size_t i) {
         const size_t cnt = std::min<size_t>(nl[i], C);
-        auto key = [&](const Item &it) {
-            return BoardKey{{it.b[0], it.b[1], width == 4 ? 0u : it.b[2], width == 4 ? 0u : it.b[3]}};
-        };
-        auto F = [](const Item &it) { return std::max((int)it.x0, meta_g(it.meta) + meta_h(it.meta)); };
-        best.clear();
-        keep.assign(cnt, 0);
+        std::vector<Entry> v(cnt);
         for (size_t j = 0; j < cnt; ++j) {
             const Item &it = items[i * C + j];
-            auto b = best.emplace(key(it), j);
-            if (!b.second) {
-                const Item &o = items[i * C + b.first->second];
-                if (std::make_pair(F(it), meta_prev(it.meta)) < std::make_pair(F(o), meta_prev(o.meta)))
-                    b.first->second = j;
+            v[j].k = BoardKey{{it.b[0], it.b[1], width == 4 ? 0u : it.b[2], width == 4 ? 0u : it.b[3]}};
+            v[j].f = std::max((int)it.x0, meta_g(it.meta) + meta_h(it.meta));
+            v[j].prev = meta_prev(it.meta);
+            v[j].j = (uint32_t)j;
+        }
+        std::sort(v.begin(), v.end());
+        std::vector<char> keep(cnt, 0);
+        SeenBoards &sn = seen[i];
+        SeenBoards merged;
+        merged.reserve(sn.size() + cnt);
+        size_t a = 0;
+        for (size_t x = 0; x < cnt; ++x) {
+            if (x > 0 && v[x].k == v[x - 1].k) continue; // not the board's best copy
+            while (a < sn.size() && sn[a].first < v[x].k) merged.push_back(sn[a++]);
+            if (a < sn.size() && sn[a].first == v[x].k) {
+                if (sn[a].second <= v[x].f) continue; // a shallower kept copy covers it
+                merged.emplace_back(v[x].k, v[x].f);
+                ++a;
+            } else {
+                merged.emplace_back(v[x].k, v[x].f);
             }
+            keep[v[x].j] = 1;
         }
-        for (auto &kv : best) {
-            const Item &it = items[i * C + kv.second];
-            const int f = F(it);
-            auto sf = seen[i].find(kv.first);
-            if (sf != seen[i].end() && sf->second <= f) continue; // a shallower kept copy covers it
-            if (sf == seen[i].end())
-                seen[i].emplace(kv.first, f);
-            else
-                sf->second = f;
-            keep[kv.second] = 1;
-        }
+        while (a < sn.size()) merged.push_back(sn[a++]);
+        sn.swap(merged);
         size_t out = 0;
         for (size_t j = 0; j < cnt; ++j) {
             if (!keep[j]) continue;
@@ -1078,7 +1091,14 @@ static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur,
             ++out;
         }
         nl[i] = (unsigned)out;
-    }
+    };
+    const size_t nth = std::min<size_t>(N, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < nth; ++t)
+        pool.emplace_back([&, t]() {
+            for (size_t i = t; i < N; i += nth) one(i);
+        });
+    for (auto &th : pool) th.join();
     e = hipMemcpyAsync(ctx->nlvl[cur].p, nl.data(), N * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(ctx->lvl[cur].p, items.data(), N * C * sizeof(Item), hipMemcpyHostToDevice, s);
     if (e == hipSuccess && paths) e = hipMemcpyAsync(ctx->lpath[cur].p, lp.data(), N * C * 8, hipMemcpyHostToDevice, s);
