@@ -99,10 +99,19 @@ def test_cli_reference_format(tmp_path, golden, ref_results):
     assert len(csv.read_text().split()) == len(fx)
     r2 = subprocess.run([exe, "--batch", *paths], capture_output=True, text=True, timeout=120)
     assert r2.returncode == 0 and r2.stdout.splitlines() == res_lines
-    # one iteration at a time: the same lines
+    # one iteration at a time, and rounds without a time limit: the same lines
     r3 = subprocess.run([exe, "--batch", "--no-lookahead", *paths], capture_output=True, text=True,
                         timeout=120)
     assert r3.returncode == 0 and r3.stdout.splitlines() == res_lines
+    r4 = subprocess.run([exe, "--batch", "--round-time", "-1", *paths], capture_output=True, text=True,
+                        timeout=120)
+    assert r4.returncode == 0 and r4.stdout.splitlines() == res_lines
+    # duplicate pruning: the same lengths, no more nodes
+    r5 = subprocess.run([exe, "--batch", "--dedup", *paths], capture_output=True, text=True, timeout=120)
+    assert r5.returncode == 0
+    for a, b in zip(res_lines, r5.stdout.splitlines()):
+        ta, tb = a.split(), b.split()
+        assert ta[:2] == tb[:2] and int(tb[2]) <= int(ta[2])
 
 
 def test_cli_paths(tmp_path, golden):
