@@ -35,6 +35,7 @@ extern "C" {
 #define SPZ_E_UNSOLVABLE -6
 #define SPZ_E_STALL -7    /* internal: work vanished with instances active */
 #define SPZ_E_TIMEOUT -8  /* wall-clock limit (env SPZ_TIMEOUT_S, default 900) */
+#define SPZ_E_MISMATCH -9 /* spz_autotune: a candidate configuration changed the result */
 
 /* heuristics */
 #define SPZ_H_MANHATTAN 0 /* src/24puzzle.cc:89-108 */
@@ -114,6 +115,17 @@ typedef struct spz_config {
                                 lengths are unchanged; node counts become
                                 smaller than the reference's, so parity runs
                                 leave it off (default 0) */
+    int32_t resplit;         /* heavy-node re-split under the SPZ_LB_PREVLOAD
+                                and SPZ_LB_LEARNED policies: when an iteration
+                                starts, a frontier node whose estimated
+                                subtree (previous iteration's count, or the
+                                prediction) exceeds resplit x the mean over
+                                the nodes that fit is queued as its children
+                                that fit (beyond 8 x that: grandchildren), the
+                                reference's static load balancing
+                                (src/15puzzle_block_parallel.cu:490-563,
+                                divide_root_set :325-375).  Counts and lengths
+                                are unchanged.  0 = off (default) */
 } spz_config;
 
 typedef struct spz_result {
@@ -141,11 +153,14 @@ void spz_get_config(spz_ctx *ctx, spz_config *cfg);
 /* Autotuner (no reference counterpart; the reference's shared-memory sweep
  * was done by hand, main.tex:742): coordinate descent over workgroup size
  * 64/128/256, root-split frontier target (= split depth), LDS stack depth,
- * round budget, lookahead on/off and the round time limit on/off, timing
- * spz_solve_batch on the sample boards.  Every
- * candidate must reproduce the first one's lengths and per-iteration
- * counts (SPZ_E_STALL otherwise).  The context keeps the fastest
- * configuration, also returned in *best. */
+ * round budget, lookahead on/off, the round time limit on/off and, under
+ * SPZ_LB_PREVLOAD or SPZ_LB_LEARNED, the re-split factor 0/1/2/4, timing
+ * spz_solve_batch on the sample boards.  Every candidate must reproduce the
+ * first one's lengths, thresholds and per-iteration counts (in first-solution
+ * mode all but the final, partial count); otherwise SPZ_E_MISMATCH.  With
+ * duplicate pruning the frontier target stays fixed (pruned counts depend on
+ * the split depth).  The context keeps the fastest configuration, also
+ * returned in *best. */
 int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_iters, int32_t passes,
                  spz_config *best, double *best_ms, int32_t *tried);
 
@@ -153,9 +168,18 @@ int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_ite
  * the goal (blank at cell 0).  Replaces loading Korf's pat24.*.tab files
  * (src/24puzzle_with_pdb.cc:77-112; src/24puzzle_block_parallel_with_pdb.cu:
  * 164-230,943-944).  cache_path: if non-NULL and the file exists, the two
- * canonical tables are read from it instead; if it does not exist they are
- * written there after the build. */
+ * canonical tables are read from it instead (a file of this library: magic,
+ * sizes and per-table checksums are checked, SPZ_E_IO if any differs); if it
+ * does not exist they are written there after the build (through a
+ * temporary file renamed into place). */
 int spz_build_pdb(spz_ctx *ctx, const char *cache_path);
+
+/* Load the reference's two table files instead of building them: the
+ * pattern-database input of src/24puzzle_with_pdb.cc:77-112 (PatternDataBase::
+ * input reading ../pdb/pat24.1256712.tab and ../pdb/pat24.34891314.tab).  Each
+ * file must be exactly 127,512,000 bytes (one per placement, nested-loop
+ * order) with 0 at the goal placement; SPZ_E_IO otherwise. */
+int spz_load_pdb_tab(spz_ctx *ctx, const char *path_1256712, const char *path_34891314);
 
 /* Write the two canonical tables in the reference's .tab layout (one byte
  * per placement, nested-loop order, src/24puzzle_with_pdb.cc:87-112). */
@@ -222,7 +246,7 @@ int spz_get_samples(spz_ctx *ctx, uint8_t *boards, int32_t *thr, uint64_t *loads
  * work item (the reference leaves it out, idas_dynamic.cu:1496-1497). */
 int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens);
 
-/* Statistics of the last solve (stats must hold 24 doubles):
+/* Statistics of the last solve (stats must hold SPZ_STATS_LEN doubles):
  *  0 rounds run            1 solve wall ms        2 items donated
  *  3 lanes                 4,5 PDB BFS levels     6 PDB build ms
  *  7 stack-overflow items  8 round-kernel ms (sum, when time_kernels)
@@ -235,8 +259,11 @@ int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens);
  *  20-22 with SPZ_PROFILE=1 (PDB): moves into a sibling (2 table reads each),
  *        children's first-partition reads, second-partition reads
  *  23 iterations started as lookahead
+ *  24 frontier nodes re-split (config resplit)  25 the queue items they became
+ *  26-31 reserved (0)
  * With env SPZ_TRACE=1 each poll of the round loop prints elapsed ms, busy
  * lanes, active instances and queued items to stderr. */
+#define SPZ_STATS_LEN 32
 int spz_last_stats(spz_ctx *ctx, double *stats);
 
 const char *spz_last_error(spz_ctx *ctx);

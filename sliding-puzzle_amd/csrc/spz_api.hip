@@ -19,7 +19,7 @@ hipError_t launch_round(int width, int heur, const KParams &p, const PdbGeo *geo
                         hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 size_t round_lds_bytes(int width, int maxd, int wg, int heur);
 const void *round_kernel_ptr(int width, int heur);
-hipError_t launch_restart(const KParams &p, hipStream_t s);
+hipError_t launch_restart(int width, int heur, const KParams &p, const PdbGeo *geo, hipStream_t s);
 hipError_t launch_spec(const KParams &p, hipStream_t s);
 hipError_t launch_control(const KParams &p, hipStream_t s);
 hipError_t launch_split_level(int width, int heur, const KParams &p, const PdbGeo *geo, const Item *lvl_in,
@@ -36,16 +36,21 @@ hipError_t pdb_spread(const uint8_t *compact, uint8_t *direct, hipStream_t s);
 hipError_t launch_gather_probe(const uint8_t *t0, const uint8_t *t1, int blocks, int per_thread, uint64_t seed,
                                uint32_t *out, hipStream_t s);
 constexpr uint64_t kPlacementsHost = 127512000ull;
+uint32_t pdb_goal_rank(int pattern);
 } // namespace spz
 
 using namespace spz;
 
 namespace {
 
-// device buffer that grows on demand
+// device buffer that grows on demand; freed when it goes out of scope
 struct DBuf {
     void *p = nullptr;
     size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
     hipError_t ensure(size_t bytes) {
         if (bytes <= n) return hipSuccess;
         if (p) (void)hipFree(p); // the old contents are not needed; a failed free leaks at worst
@@ -63,6 +68,30 @@ struct DBuf {
     template <class T> T *as() const { return (T *)p; }
 };
 
+// pinned host buffer that grows on demand; freed when it goes out of scope
+struct HBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    HBuf() = default;
+    HBuf(const HBuf &) = delete;
+    HBuf &operator=(const HBuf &) = delete;
+    ~HBuf() { release(); }
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= n) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
 } // namespace
 
 struct spz_ctx {
@@ -73,7 +102,7 @@ struct spz_ctx {
     int wg = 256;
     double wall_ticks_per_us = 0; // device wall clock (wall_clock64) rate
     std::string err;
-    double stats[24] = {0};
+    double stats[SPZ_STATS_LEN] = {0};
     DBuf prof;             // lane-state counters (env SPZ_PROFILE=1)
     DBuf wtime;            // per-wave clocks of one round (env SPZ_WAVE_TIMES)
     double timeout_s = 0;
@@ -101,6 +130,7 @@ struct spz_ctx {
     std::vector<int> paths_slot; // their final iteration's slot
     MlpState mlp;
     bool mlp_alloced = false;
+    HBuf dd_items, dd_paths; // staging of the duplicate-pruning pass
 };
 
 static int fail(spz_ctx *c, int code, const char *fmt, ...) {
@@ -240,6 +270,7 @@ static int normalise_config(const spz_config *in, spz_config *c, int *wg) {
         c->lds_stack_depth = c->width == 4 ? Geo<4>::DEFD : (c->heuristic == SPZ_H_PDB666 ? kDefDepthPdb : Geo<5>::DEFD);
     if (c->lds_stack_depth < 8 || c->lds_stack_depth > Geo<5>::MAXD) return SPZ_E_ARG;
     if (c->lb_policy < SPZ_LB_UNIFORM || c->lb_policy > SPZ_LB_PREVLOAD) return SPZ_E_ARG;
+    if (c->resplit < 0 || c->resplit > 1024) return SPZ_E_ARG;
     return SPZ_OK;
 }
 
@@ -322,12 +353,22 @@ void spz_get_config(spz_ctx *ctx, spz_config *cfg) {
 }
 
 // Coordinate descent over workgroup size, root-split target, LDS stack depth,
-// round budget, lookahead on/off and the round time limit on/off on a sample
-// batch; every candidate must give the same lengths and counts.  The context
-// is left configured with the winner.
+// round budget, lookahead on/off, the round time limit on/off and (under a
+// policy with estimates) the heavy-node re-split factor on a sample batch;
+// every candidate must give the same result.  The context is left configured
+// with the winner.
+//
+// What "the same result" is depends on the configuration:
+//  - lengths and thresholds always;
+//  - all-solutions mode: every per-iteration count;
+//  - first-solution mode: every count but the last iteration's, which stops
+//    at a goal and so depends on the schedule;
+//  - duplicate pruning: the pruned counts depend on the root-split depth, so
+//    the frontier target is not varied (the other dimensions keep counts).
 int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_iters, int32_t passes,
                  spz_config *best, double *best_ms, int32_t *tried) {
     if (!ctx || !boards || n <= 0 || !best) return SPZ_E_ARG;
+    if (max_iters < 1 || max_iters > kMaxIters) return fail(ctx, SPZ_E_ARG, "max_iters must be in 1..%d", kMaxIters);
     if (passes <= 0) passes = 1;
     std::vector<spz_result> ref((size_t)n), cur((size_t)n);
     std::vector<uint64_t> ic_ref((size_t)n * max_iters), ic((size_t)n * max_iters);
@@ -338,6 +379,8 @@ int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_ite
     const int deps_w4[4] = {48, 64, 80, 112}, deps_w5[4] = {64, 80, 96, 128};
     const int *deps = base.width == 4 ? deps_w4 : deps_w5;
     const int buds[4] = {256, 512, 1024, 2048};
+    const int splits[4] = {0, 1, 2, 4};
+    const bool first = base.mode == SPZ_MODE_FIRST;
     auto run = [&](const spz_config &c, double *ms, bool check) -> int {
         int rc = spz_reconfigure(ctx, &c);
         if (rc) return rc;
@@ -347,11 +390,18 @@ int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_ite
         *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
         if (rc) return rc;
         if (check) {
-            for (int i = 0; i < n; ++i)
-                if (cur[i].opt_len != ref[i].opt_len || cur[i].total_nodes != ref[i].total_nodes)
-                    return fail(ctx, SPZ_E_STALL, "autotune: candidate changed the result of board %d", i);
-            if (memcmp(ic.data(), ic_ref.data(), ic.size() * 8) != 0)
-                return fail(ctx, SPZ_E_STALL, "autotune: candidate changed a per-iteration count");
+            for (int i = 0; i < n; ++i) {
+                const spz_result &a = cur[i], &b = ref[i];
+                if (a.opt_len != b.opt_len || a.n_iters != b.n_iters || a.h0 != b.h0)
+                    return fail(ctx, SPZ_E_MISMATCH, "autotune: candidate changed the length of board %d", i);
+                // per-iteration counts; in first-solution mode the final one
+                // is partial and schedule-dependent
+                const int cmp = first ? a.n_iters - 1 : max_iters;
+                for (int k = 0; k < cmp; ++k)
+                    if (ic[(size_t)i * max_iters + k] != ic_ref[(size_t)i * max_iters + k])
+                        return fail(ctx, SPZ_E_MISMATCH,
+                                    "autotune: candidate changed count %d of board %d", k, i);
+            }
         }
         return SPZ_OK;
     };
@@ -365,8 +415,10 @@ int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_ite
     if (rc) return rc;
     count = 2;
     for (int pass = 0; pass < passes; ++pass) {
-        for (int dim = 0; dim < 6; ++dim) {
-            int nv = dim == 0 ? 3 : dim == 1 ? 5 : dim >= 4 ? 2 : 4;
+        for (int dim = 0; dim < 7; ++dim) {
+            if (dim == 1 && base.dedup) continue;
+            if (dim == 6 && base.lb_policy == SPZ_LB_UNIFORM) continue;
+            int nv = dim == 0 ? 3 : dim == 1 ? 5 : (dim == 4 || dim == 5) ? 2 : 4;
             spz_config winner = bc;
             for (int v = 0; v < nv; ++v) {
                 spz_config c = bc;
@@ -376,6 +428,7 @@ int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_ite
                 if (dim == 3) c.round_budget = buds[v];
                 if (dim == 4) c.lookahead = v;
                 if (dim == 5) c.round_time_us = v ? -1 : 0; // time limit (auto) or step budget only
+                if (dim == 6) c.resplit = splits[v];
                 if (memcmp(&c, &bc, sizeof c) == 0) continue;
                 c.lanes = 0;
                 double ms = 0;
@@ -418,6 +471,8 @@ void spz_destroy(spz_ctx *ctx) {
                    &ctx->listed,      &ctx->want_next,      &ctx->dead,           &ctx->prof,
                    &ctx->wtime};
     for (DBuf *b : all) b->release();
+    ctx->dd_items.release();
+    ctx->dd_paths.release();
     if (ctx->mlp_alloced) mlp_free(ctx->mlp);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -448,6 +503,84 @@ static int upload_direct(spz_ctx *ctx) {
     return SPZ_OK;
 }
 
+// PDB cache file: a 64-byte header, then the two canonical tables (file
+// order).  Header: magic "SPZPDB01", u64 bytes per table, u64 table count
+// (2), u64 checksum of each table, zero padding.  A file that is not exactly
+// that size, or whose magic, sizes or checksums do not match, is rejected.
+constexpr char kCacheMagic[8] = {'S', 'P', 'Z', 'P', 'D', 'B', '0', '1'};
+constexpr size_t kCacheHeader = 64;
+
+// checksum of a table (64-bit multiply-xor over 8-byte words; detects
+// truncation and corruption, not tampering)
+static uint64_t table_checksum(const uint8_t *p, size_t n) {
+    uint64_t h = 0x9e3779b97f4a7c15ull ^ n;
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        memcpy(&w, p + i, 8);
+        h = (h ^ w) * 0xff51afd7ed558ccdull;
+        h ^= h >> 29;
+    }
+    for (; i < n; ++i) h = (h ^ p[i]) * 0xc4ceb9fe1a85ec53ull;
+    return h ^ (h >> 32);
+}
+
+// the two canonical tables from host memory to the device, then the direct
+// tables the search reads
+static int upload_compact(spz_ctx *ctx, const uint8_t *t0, const uint8_t *t1) {
+    const uint8_t *t[2] = {t0, t1};
+    for (int k = 0; k < 2; ++k)
+        HIPCHK(hipMemcpy(ctx->pdb_compact[k].p, t[k], kPlacementsHost, hipMemcpyHostToDevice));
+    return upload_direct(ctx);
+}
+
+static int read_cache(spz_ctx *ctx, const char *path, FILE *f) {
+    std::vector<uint8_t> buf(2 * kPlacementsHost);
+    uint8_t hdr[kCacheHeader];
+    const size_t got = fread(hdr, 1, kCacheHeader, f);
+    const size_t got2 = got == kCacheHeader ? fread(buf.data(), 1, buf.size(), f) : 0;
+    const bool at_end = got2 == buf.size() && fgetc(f) == EOF;
+    fclose(f);
+    uint64_t v[4];
+    memcpy(v, hdr + 8, sizeof v);
+    if (got != kCacheHeader || memcmp(hdr, kCacheMagic, 8) != 0 || v[0] != kPlacementsHost || v[1] != 2)
+        return fail(ctx, SPZ_E_IO, "%s is not a PDB cache file of this library", path);
+    if (!at_end) return fail(ctx, SPZ_E_IO, "PDB cache %s has the wrong size", path);
+    for (int k = 0; k < 2; ++k)
+        if (table_checksum(buf.data() + k * kPlacementsHost, kPlacementsHost) != v[2 + k])
+            return fail(ctx, SPZ_E_IO, "PDB cache %s: table %d fails its checksum", path, k);
+    return upload_compact(ctx, buf.data(), buf.data() + kPlacementsHost);
+}
+
+// written to a temporary file beside the target, then renamed over it, so a
+// reader never sees a partial cache
+static int write_cache(spz_ctx *ctx, const char *path) {
+    std::vector<uint8_t> buf(2 * kPlacementsHost);
+    for (int k = 0; k < 2; ++k)
+        HIPCHK(hipMemcpy(buf.data() + k * kPlacementsHost, ctx->pdb_compact[k].p, kPlacementsHost,
+                         hipMemcpyDeviceToHost));
+    uint8_t hdr[kCacheHeader];
+    memset(hdr, 0, sizeof hdr);
+    memcpy(hdr, kCacheMagic, 8);
+    const uint64_t v[4] = {kPlacementsHost, 2, table_checksum(buf.data(), kPlacementsHost),
+                           table_checksum(buf.data() + kPlacementsHost, kPlacementsHost)};
+    memcpy(hdr + 8, v, sizeof v);
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return fail(ctx, SPZ_E_IO, "cannot write PDB cache %s", tmp.c_str());
+    const bool ok = fwrite(hdr, 1, sizeof hdr, f) == sizeof hdr && fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    const bool closed = fclose(f) == 0;
+    if (!ok || !closed) {
+        remove(tmp.c_str());
+        return fail(ctx, SPZ_E_IO, "short write to %s", tmp.c_str());
+    }
+    if (rename(tmp.c_str(), path) != 0) {
+        remove(tmp.c_str());
+        return fail(ctx, SPZ_E_IO, "cannot rename %s to %s", tmp.c_str(), path);
+    }
+    return SPZ_OK;
+}
+
 int spz_build_pdb(spz_ctx *ctx, const char *cache_path) {
     if (!ctx) return SPZ_E_ARG;
     if (ctx->cfg.width != 5) return fail(ctx, SPZ_E_ARG, "PDBs exist for the 24-puzzle only");
@@ -455,18 +588,7 @@ int spz_build_pdb(spz_ctx *ctx, const char *cache_path) {
     for (int t = 0; t < 2; ++t) HIPCHK(ctx->pdb_compact[t].ensure(kPlacementsHost));
     if (cache_path) {
         FILE *f = fopen(cache_path, "rb");
-        if (f) {
-            std::vector<uint8_t> buf(kPlacementsHost);
-            for (int t = 0; t < 2; ++t) {
-                if (fread(buf.data(), 1, buf.size(), f) != buf.size()) {
-                    fclose(f);
-                    return fail(ctx, SPZ_E_IO, "short PDB cache file %s", cache_path);
-                }
-                HIPCHK(hipMemcpy(ctx->pdb_compact[t].p, buf.data(), buf.size(), hipMemcpyHostToDevice));
-            }
-            fclose(f);
-            return upload_direct(ctx);
-        }
+        if (f) return read_cache(ctx, cache_path, f);
     }
     DBuf work, cntb;
     hipError_t e = work.ensure(3ull * kPlacementsHost * 4);
@@ -491,20 +613,34 @@ int spz_build_pdb(spz_ctx *ctx, const char *cache_path) {
     if (e != hipSuccess) return fail(ctx, SPZ_E_HIP, "PDB BFS: %s", hipGetErrorString(e));
     int rc = upload_direct(ctx);
     if (rc != SPZ_OK) return rc;
-    if (cache_path) {
-        FILE *f = fopen(cache_path, "wb");
-        if (!f) return fail(ctx, SPZ_E_IO, "cannot write PDB cache %s", cache_path);
-        std::vector<uint8_t> buf(kPlacementsHost);
-        for (int t = 0; t < 2; ++t) {
-            HIPCHK(hipMemcpy(buf.data(), ctx->pdb_compact[t].p, buf.size(), hipMemcpyDeviceToHost));
-            if (fwrite(buf.data(), 1, buf.size(), f) != buf.size()) {
-                fclose(f);
-                return fail(ctx, SPZ_E_IO, "short write to %s", cache_path);
-            }
-        }
-        fclose(f);
-    }
+    if (cache_path) return write_cache(ctx, cache_path);
     return SPZ_OK;
+}
+
+// The reference's table files (src/24puzzle_with_pdb.cc:77-112): each is
+// exactly 127,512,000 bytes, one byte per placement of the pattern's six
+// tiles in nested-loop order.  Besides the size, the entry of the goal
+// placement (every pattern tile home) must be 0.
+int spz_load_pdb_tab(spz_ctx *ctx, const char *path_1256712, const char *path_34891314) {
+    if (!ctx || !path_1256712 || !path_34891314) return SPZ_E_ARG;
+    if (ctx->cfg.width != 5) return fail(ctx, SPZ_E_ARG, "PDBs exist for the 24-puzzle only");
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    for (int t = 0; t < 2; ++t) HIPCHK(ctx->pdb_compact[t].ensure(kPlacementsHost));
+    std::vector<uint8_t> buf(2 * kPlacementsHost);
+    const char *paths[2] = {path_1256712, path_34891314};
+    for (int t = 0; t < 2; ++t) {
+        FILE *f = fopen(paths[t], "rb");
+        if (!f) return fail(ctx, SPZ_E_IO, "cannot open %s", paths[t]);
+        const size_t got = fread(buf.data() + t * kPlacementsHost, 1, kPlacementsHost, f);
+        const bool at_end = got == kPlacementsHost && fgetc(f) == EOF;
+        fclose(f);
+        if (!at_end) return fail(ctx, SPZ_E_IO, "%s is not %llu bytes", paths[t], (unsigned long long)kPlacementsHost);
+        const uint32_t goal = pdb_goal_rank(t);
+        if (buf[t * kPlacementsHost + goal] != 0)
+            return fail(ctx, SPZ_E_IO, "%s: the goal placement's entry is not 0", paths[t]);
+    }
+    ctx->have_pdb = false;
+    return upload_compact(ctx, buf.data(), buf.data() + kPlacementsHost);
 }
 
 int spz_get_pdb(spz_ctx *ctx, int32_t pattern, uint8_t *out) {
@@ -532,12 +668,16 @@ int spz_export_pdb_tab(spz_ctx *ctx, const char *p0, const char *p1) {
 
 // -------------------------------------------------------------- solve ---
 // Duplicate pruning in the root split (config dedup).  Before a level is
-// expanded, a node is dropped when an earlier kept node of its instance's top
-// tree holds the same board with a path-max f no larger than its own: that
-// node reaches everything the dropped one would within any threshold (its g
-// is no larger, the heuristic of a board is the same), so optimal lengths
-// stay; the dropped node's subtree is no longer counted.  Host side, between
-// split levels: it changes the set-up only, never the search kernels.
+// expanded, a node is dropped when a kept node of its instance's top tree --
+// the root, a node of a shallower level, or a copy in the same level --
+// holds the same board with a path-max f no larger than its own.  The kept
+// copy's g is no larger (levels run in depth order), so below it every path
+// of the dropped copy has a counterpart whose every f, and so path-max f, is
+// no larger: it is expanded in every iteration in which the dropped copy's
+// would be, for any admissible heuristic, consistent or not.  Optimal
+// lengths stay; the dropped node's subtree is no longer counted.  Host side,
+// between split levels: it changes the set-up only, never the search
+// kernels.
 struct BoardKey {
     uint32_t w[4];
     bool operator<(const BoardKey &o) const {
@@ -552,6 +692,10 @@ struct BoardKey {
 // an instance's kept top-tree boards with the smallest path-max f kept for
 // each, sorted by board
 using SeenBoards = std::vector<std::pair<BoardKey, int>>;
+
+static BoardKey board_key(int width, const Item &it) {
+    return BoardKey{{it.b[0], it.b[1], width == 4 ? 0u : it.b[2], width == 4 ? 0u : it.b[3]}};
+}
 
 static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur, bool paths,
                               std::vector<SeenBoards> &seen, hipStream_t s);
@@ -601,6 +745,7 @@ static KParams base_params(spz_ctx *ctx, int n, int max_iters, int capl) {
     p.c_origin = ctx->c_origin.as<uint32_t>();
     p.load = ctx->load.as<unsigned long long>();
     p.lb = ctx->cfg.lb_policy;
+    p.resplit = ctx->cfg.resplit;
     p.pred = ctx->pred.as<float>();
     p.samples = ctx->samples.as<Sample>();
     p.n_samples = ctx->nsamp.as<unsigned>();
@@ -765,6 +910,9 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     auto t_split = std::chrono::steady_clock::now();
     int cur = 0;
     std::vector<SeenBoards> seen(ctx->cfg.dedup ? N : 0);
+    // the closed list starts with the root (src_horie/idas_smem.cu:1068)
+    for (size_t i = 0; i < seen.size(); ++i)
+        seen[i].emplace_back(board_key(w, roots[i]), meta_h(roots[i].meta));
     for (int depth = 0; depth <= max_depth; ++depth) {
         if (ctx->cfg.dedup && depth > 0) HIPCHK(dedup_level(ctx, n, capl, w, cur, paths, seen, s));
         HIPCHK(hipMemsetAsync(ctx->nlvl[cur ^ 1].p, 0, N * 4, s));
@@ -842,7 +990,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         HIPCHK(hipStreamSynchronize(s));
     }
 
-    HIPCHK(launch_restart(p, s));
+    HIPCHK(launch_restart(w, heur, p, ctx->geo_d.as<PdbGeo>(), s));
     HIPCHK(launch_control(p, s));
     auto t_rounds = std::chrono::steady_clock::now();
     ctx->stats[13] = std::chrono::duration<double, std::milli>(t_rounds - t_start).count();
@@ -875,7 +1023,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
                 e = launch_round(w, heur, p, ctx->geo_d.as<PdbGeo>(), ctx->wg, s, timing ? ev[2 * k] : nullptr,
                                  timing ? ev[2 * k + 1] : nullptr);
             if (e == hipSuccess && p.lookahead) e = launch_spec(p, s);
-            if (e == hipSuccess) e = launch_restart(p, s);
+            if (e == hipSuccess) e = launch_restart(w, heur, p, ctx->geo_d.as<PdbGeo>(), s);
             if (e == hipSuccess) e = launch_control(p, s);
         }
         if (e == hipSuccess) e = hipMemcpyAsync(h_ctl, ctx->ctl.p, 64 * 4, hipMemcpyDeviceToHost, s);
@@ -966,6 +1114,8 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     ctx->stats[3] = (double)lanes;
     ctx->stats[7] = (double)h_ctl[8];
     ctx->stats[23] = (double)h_ctl[13];
+    ctx->stats[24] = (double)h_ctl[14];
+    ctx->stats[25] = (double)h_ctl[15];
     (void)hipHostFree(h_ctl);
 
     std::vector<int> opt(N), its(N);
@@ -1027,11 +1177,23 @@ static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur,
                               std::vector<SeenBoards> &seen, hipStream_t s) {
     const size_t N = (size_t)n, C = (size_t)capl;
     std::vector<unsigned> nl(N);
-    std::vector<Item> items(N * C);
-    std::vector<unsigned long long> lp(paths ? N * C : 0);
     hipError_t e = hipMemcpyAsync(nl.data(), ctx->nlvl[cur].p, N * 4, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(items.data(), ctx->lvl[cur].p, N * C * sizeof(Item), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess && paths) e = hipMemcpyAsync(lp.data(), ctx->lpath[cur].p, N * C * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    // only the first max_i nl[i] items of each instance's region travel, in
+    // one strided copy each way, through pinned staging buffers
+    size_t width_items = 0;
+    for (size_t i = 0; i < N; ++i) width_items = std::max<size_t>(width_items, std::min<size_t>(nl[i], C));
+    if (width_items == 0) return hipSuccess;
+    if ((e = ctx->dd_items.ensure(N * width_items * sizeof(Item))) != hipSuccess) return e;
+    if (paths && (e = ctx->dd_paths.ensure(N * width_items * 8)) != hipSuccess) return e;
+    Item *items = (Item *)ctx->dd_items.p;
+    unsigned long long *lp = (unsigned long long *)ctx->dd_paths.p;
+    const size_t W = width_items;
+    e = hipMemcpy2DAsync(items, W * sizeof(Item), ctx->lvl[cur].p, C * sizeof(Item), W * sizeof(Item), N,
+                         hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && paths)
+        e = hipMemcpy2DAsync(lp, W * 8, ctx->lpath[cur].p, C * 8, W * 8, N, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
     // within a level the copy kept is the one with the smallest (path-max f,
@@ -1055,8 +1217,8 @@ static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur,
         const size_t cnt = std::min<size_t>(nl[i], C);
         std::vector<Entry> v(cnt);
         for (size_t j = 0; j < cnt; ++j) {
-            const Item &it = items[i * C + j];
-            v[j].k = BoardKey{{it.b[0], it.b[1], width == 4 ? 0u : it.b[2], width == 4 ? 0u : it.b[3]}};
+            const Item &it = items[i * W + j];
+            v[j].k = board_key(width, it);
             v[j].f = std::max((int)it.x0, meta_g(it.meta) + meta_h(it.meta));
             v[j].prev = meta_prev(it.meta);
             v[j].j = (uint32_t)j;
@@ -1085,8 +1247,8 @@ static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur,
         for (size_t j = 0; j < cnt; ++j) {
             if (!keep[j]) continue;
             if (out != j) {
-                items[i * C + out] = items[i * C + j];
-                if (paths) lp[i * C + out] = lp[i * C + j];
+                items[i * W + out] = items[i * W + j];
+                if (paths) lp[i * W + out] = lp[i * W + j];
             }
             ++out;
         }
@@ -1100,8 +1262,11 @@ static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur,
         });
     for (auto &th : pool) th.join();
     e = hipMemcpyAsync(ctx->nlvl[cur].p, nl.data(), N * 4, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(ctx->lvl[cur].p, items.data(), N * C * sizeof(Item), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && paths) e = hipMemcpyAsync(ctx->lpath[cur].p, lp.data(), N * C * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(ctx->lvl[cur].p, C * sizeof(Item), items, W * sizeof(Item), W * sizeof(Item), N,
+                             hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && paths)
+        e = hipMemcpy2DAsync(ctx->lpath[cur].p, C * 8, lp, W * 8, W * 8, N, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e;
 }
@@ -1269,30 +1434,33 @@ int spz_mlp_train(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const
     HIPCHK(hipSetDevice(ctx->cfg.device));
     const int w = ctx->cfg.width;
     hipStream_t s = ctx->stream;
-    // all rows to the device once; batches are gathered by a permutation
+    // all rows, thresholds and targets to the device once; each epoch uploads
+    // one permutation and its batches are gathered on the device, so the
+    // host waits once per epoch (for the epoch's summed batch losses)
     std::vector<Item> it((size_t)n);
     for (int i = 0; i < n; ++i) {
         if (!is_solvable(w, boards + (size_t)i * w * w))
             return fail(ctx, SPZ_E_ARG, "board %d is not a solvable permutation", i);
         pack_root(w, boards + (size_t)i * w * w, i, &it[i]);
     }
-    DBuf items, bitems, bthr, btgt;
+    DBuf items, thr_d, tgt_d, perm_d, btgt, eloss;
     HIPCHK(items.ensure((size_t)n * sizeof(Item)));
+    HIPCHK(thr_d.ensure((size_t)n * 4));
+    HIPCHK(tgt_d.ensure((size_t)n * 4));
+    HIPCHK(perm_d.ensure((size_t)n * 4));
+    HIPCHK(btgt.ensure((size_t)batch * 4));
+    HIPCHK(eloss.ensure((size_t)epochs * 4));
     HIPCHK(hipMemcpyAsync(items.p, it.data(), (size_t)n * sizeof(Item), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(thr_d.p, thr, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(tgt_d.p, target, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(eloss.p, 0, (size_t)epochs * 4, s));
     KParams p = base_params(ctx, 0, 1, 1);
     HIPCHK(launch_eval_h(w, ctx->cfg.heuristic, p, ctx->geo_d.as<PdbGeo>(), items.as<Item>(), n, s));
-    HIPCHK(hipMemcpyAsync(it.data(), items.p, (size_t)n * sizeof(Item), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(bitems.ensure((size_t)batch * sizeof(Item)));
-    HIPCHK(bthr.ensure((size_t)batch * 4));
-    HIPCHK(btgt.ensure((size_t)batch * 4));
     HIPCHK(mlp_reserve(ctx->mlp, (size_t)batch));
     std::vector<int> perm((size_t)n);
     for (int i = 0; i < n; ++i) perm[i] = i;
     uint64_t x = seed;
-    std::vector<Item> bi((size_t)batch);
-    std::vector<int> bt((size_t)batch);
-    std::vector<float> bg((size_t)batch);
+    std::vector<int> nbatches((size_t)epochs, 0);
     for (int ep = 0; ep < epochs; ++ep) {
         for (int i = n - 1; i > 0; --i) { // Fisher-Yates with splitmix64
             uint64_t z = (x += 0x9e3779b97f4a7c15ULL);
@@ -1301,30 +1469,25 @@ int spz_mlp_train(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const
             z ^= z >> 31;
             std::swap(perm[i], perm[(int)(z % (uint64_t)(i + 1))]);
         }
-        double sum = 0;
-        int nb = 0;
+        // the upload is ordered before this epoch's kernels on the stream; the
+        // host vector is reshuffled only after the epoch's synchronisation
+        HIPCHK(hipMemcpyAsync(perm_d.p, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+        float *el = eloss.as<float>() + ep;
         for (int b0 = 0; b0 + batch <= n || (b0 == 0 && n < batch); b0 += batch) {
-            int nbat = std::min(batch, n - b0);
-            for (int j = 0; j < nbat; ++j) {
-                bi[j] = it[perm[b0 + j]];
-                bt[j] = thr[perm[b0 + j]];
-                bg[j] = target[perm[b0 + j]];
-            }
-            HIPCHK(hipMemcpyAsync(bitems.p, bi.data(), (size_t)nbat * sizeof(Item), hipMemcpyHostToDevice, s));
-            HIPCHK(hipMemcpyAsync(bthr.p, bt.data(), (size_t)nbat * 4, hipMemcpyHostToDevice, s));
-            HIPCHK(hipMemcpyAsync(btgt.p, bg.data(), (size_t)nbat * 4, hipMemcpyHostToDevice, s));
-            HIPCHK(mlp_features_items(ctx->mlp, w, bitems.as<Item>(), bthr.as<int>(), nbat, s));
+            const int nbat = std::min(batch, n - b0);
+            HIPCHK(mlp_features_gather(ctx->mlp, w, items.as<Item>(), thr_d.as<int>(), tgt_d.as<float>(),
+                                       perm_d.as<int>() + b0, nbat, btgt.as<float>(), s));
             HIPCHK(mlp_forward(ctx->mlp, nbat, s));
-            HIPCHK(mlp_backward(ctx->mlp, btgt.as<float>(), nbat, s));
+            HIPCHK(mlp_backward(ctx->mlp, btgt.as<float>(), nbat, s, el));
             HIPCHK(mlp_adam_step(ctx->mlp, lr, s));
-            float l = 0;
-            HIPCHK(hipMemcpyAsync(&l, ctx->mlp.loss, 4, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            sum += l;
-            ++nb;
+            ++nbatches[ep];
         }
-        if (epoch_loss) epoch_loss[ep] = nb ? (float)(sum / nb) : 0.f;
+        HIPCHK(hipStreamSynchronize(s));
     }
+    std::vector<float> sums((size_t)epochs);
+    HIPCHK(hipMemcpy(sums.data(), eloss.p, (size_t)epochs * 4, hipMemcpyDeviceToHost));
+    if (epoch_loss)
+        for (int ep = 0; ep < epochs; ++ep) epoch_loss[ep] = nbatches[ep] ? sums[ep] / (float)nbatches[ep] : 0.f;
     return SPZ_OK;
 }
 

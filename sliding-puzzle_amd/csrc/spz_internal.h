@@ -34,7 +34,9 @@ constexpr int kNoGoal = 0x7fffffff; // found-depth sentinel
 
 // A search node in the work queue (32 bytes).  The board is packed:
 //  width 4: b[0] = row0 | row1<<16, b[1] = row2 | row3<<16 (4-bit cells)
-//  width 5: b[r] = row r (25 bits) | 7 bits of row 4 at bit 25 (r = 0..3)
+//  width 5: b[0..1] = u64 of cells 0-11 (5 bits each, bits 0-59) with the low
+//  4 bits of cell 24 at bits 60-63; b[2..3] = u64 of cells 12-23 with the high
+//  bit of cell 24 at bit 60
 // meta = blank | g<<8 | h<<16 | prev<<24; x0 = largest f on the root path
 // (root split), x1 = index of the frontier node the item descends from.
 struct alignas(16) Item {
@@ -108,6 +110,8 @@ struct KParams {
                                  // 7 donated, 8 overflow exports, 9/10 big/small
                                  // budget, 12 busy at last round end,
                                  // 13 lookahead iterations started,
+                                 // 14/15 frontier nodes re-split / the items
+                                 // they became,
                                  // 19 donate_per_lane of lanes on lookahead
                                  // iterations, 34 how many are busy (own
                                  // cache line: every wave adds to it)
@@ -123,6 +127,7 @@ struct KParams {
     // current iteration, dispatch policy, learned predictions, sample log
     unsigned long long *load;    // [n_inst][capl]
     int lb;                      // 0 frontier order, 1 learned, 2 previous load
+    int resplit;                 // heavy-node re-split factor (0: off)
     const float *pred;           // [n_inst][capl][kPredK] predicted log2(1+load)
     Sample *samples;
     unsigned *n_samples;

@@ -46,6 +46,7 @@ struct MlpState {
     float *y = nullptr;         // [cap]
     float *dz = nullptr;        // [cap][256] scratch gradient
     uint16_t *dzb = nullptr;    // bf16 copy
+    float *part = nullptr;      // [cap/64][256] partial column sums
     float *loss = nullptr;      // [1]
     uint16_t *w2frag = nullptr; // W2 in MFMA B-fragment order (prediction)
 };
@@ -61,8 +62,13 @@ hipError_t mlp_init_random(MlpState &m, uint64_t seed, hipStream_t s);
 hipError_t mlp_features_items(MlpState &m, int width, const Item *items, const int *thr, int n, hipStream_t s);
 // forward over rows [0,n) of the feature buffer; y in m.y
 hipError_t mlp_forward(MlpState &m, int n, hipStream_t s);
-// loss = mean (y - target)^2 and the full gradient into m.grad
-hipError_t mlp_backward(MlpState &m, const float *target, int n, hipStream_t s);
+// loss = mean (y - target)^2 (into m.loss, or added to *loss_acc) and the
+// full gradient into m.grad
+hipError_t mlp_backward(MlpState &m, const float *target, int n, hipStream_t s, float *loss_acc = nullptr);
+// features of rows items[perm[i]] at thr[perm[i]], i < n, and their targets
+// tgt[perm[i]] into tout[i] (a training batch gathered on the device)
+hipError_t mlp_features_gather(MlpState &m, int width, const Item *items, const int *thr, const float *tgt,
+                               const int *perm, int n, float *tout, hipStream_t s);
 hipError_t mlp_adam_step(MlpState &m, float lr, hipStream_t s);
 
 // Predictions at kPredThr consecutive thresholds L0, L0+2, ... per node, by

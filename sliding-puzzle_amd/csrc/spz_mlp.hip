@@ -67,17 +67,21 @@ static hipError_t gemm(const uint16_t *A, long sam, long sak, const uint16_t *B,
     return hipGetLastError();
 }
 
-// features of Items at thresholds (one wave per row, 10 columns per lane)
+// features of Items at thresholds (one wave per row, 10 columns per lane);
+// with perm, row r is built from items[perm[r]] (and its target copied)
 template <int W>
-__global__ void k_features(const Item *items, const int *thr, int n, uint16_t *x) {
+__global__ void k_features(const Item *items, const int *thr, int n, uint16_t *x, const int *perm, const float *tgt,
+                           float *tout) {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
-    const Item it = items[row];
+    const int src = perm ? perm[row] : row;
+    const Item it = items[src];
+    if (perm && lane == 0) tout[row] = tgt[src];
     Board<W> bd;
     bd.unpack(it.b);
     const int h = meta_h(it.meta);
-    const int L = thr[row];
+    const int L = thr[src];
     constexpr int N2 = W * W;
     for (int k = lane; k < kMlpIn; k += 64) {
         float v = 0.f;
@@ -126,32 +130,58 @@ __global__ void k_loss(const float *y, const float *t, int n, float *dy, float *
     if ((threadIdx.x & 63) == 0) atomicAdd(loss, e);
 }
 
-// output layer gradients: gw3[j] = sum_i h2[i][j] dy[i], gb3 = sum dy;
-// dz2[i][j] = dy[i] w3[j] [z2 > 0]
-__global__ void k_out_grad(const uint16_t *h2, const float *z2, const float *dy, const uint16_t *wbf, float *grad,
-                           float *dz, uint16_t *dzb, int n) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x; // column
-    if (j >= kMlpH) return;
-    float w3 = bf2f(wbf[kMlpOffW3 + j]);
-    float g = 0.f, gb = 0.f;
-    for (int i = 0; i < n; ++i) {
-        g += bf2f(h2[(long)i * kMlpH + j]) * dy[i];
-        gb += dy[i];
-        float v = z2[(long)i * kMlpH + j] > 0.f ? dy[i] * w3 : 0.f;
-        dz[(long)i * kMlpH + j] = v;
-        dzb[(long)i * kMlpH + j] = f2bf(v);
-    }
-    grad[kMlpOffW3 + j] = g;
-    if (j == 0) grad[kMlpOffB3] = gb;
+// output layer: dz2[i][j] = dy[i] w3[j] [z2 > 0] (fp32 and bf16), one
+// thread per element
+__global__ void k_out_dz(const float *z2, const float *dy, const uint16_t *wbf, float *dz, uint16_t *dzb, int n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * kMlpH) return;
+    const int row = (int)(i / kMlpH), j = (int)(i - (long)row * kMlpH);
+    const float v = z2[i] > 0.f ? dy[row] * bf2f(wbf[kMlpOffW3 + j]) : 0.f;
+    dz[i] = v;
+    dzb[i] = f2bf(v);
 }
 
-// column sums (bias gradients) of dz
-__global__ void k_colsum(const float *dz, int n, float *out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= kMlpH) return;
+// Column sums over n rows of 256 columns in two passes with a fixed order
+// (so training is deterministic): block b sums rows [b*kSumRows, ...) into
+// part[b][j], then k_colsum_fin adds the parts.  Weighted form: sum_i
+// bf16 a[i][j] * w[i] (the output weights' gradient, h2^T dy).
+constexpr int kSumRows = 64;
+__global__ void __launch_bounds__(256) k_colsum_part(const float *a, int n, float *part) {
+    const int j = threadIdx.x, b = blockIdx.x;
+    const int r1 = min(n, (b + 1) * kSumRows);
     float s = 0.f;
-    for (int i = 0; i < n; ++i) s += dz[(long)i * kMlpH + j];
+    for (int i = b * kSumRows; i < r1; ++i) s += a[(long)i * kMlpH + j];
+    part[(long)b * kMlpH + j] = s;
+}
+__global__ void __launch_bounds__(256) k_colsum_part_w(const uint16_t *a, const float *w, int n, float *part) {
+    const int j = threadIdx.x, b = blockIdx.x;
+    const int r1 = min(n, (b + 1) * kSumRows);
+    float s = 0.f;
+    for (int i = b * kSumRows; i < r1; ++i) s += bf2f(a[(long)i * kMlpH + j]) * w[i];
+    part[(long)b * kMlpH + j] = s;
+}
+__global__ void __launch_bounds__(256) k_colsum_fin(const float *part, int nparts, float *out) {
+    const int j = threadIdx.x;
+    float s = 0.f;
+    for (int b = 0; b < nparts; ++b) s += part[(long)b * kMlpH + j];
     out[j] = s;
+}
+// sum of n values, one block (the output bias gradient)
+__global__ void __launch_bounds__(256) k_sum(const float *v, int n, float *out) {
+    __shared__ float sh[4];
+    float s = 0.f;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += v[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+static hipError_t colsum(const float *a, int n, float *part, float *out, hipStream_t s) {
+    const int nb = (n + kSumRows - 1) / kSumRows;
+    hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)nb), dim3(kMlpH), 0, s, a, n, part);
+    hipLaunchKernelGGL(k_colsum_fin, dim3(1), dim3(kMlpH), 0, s, part, nb, out);
+    return hipGetLastError();
 }
 
 // dz1 = dh1 [z1 > 0] in place (dh1 given in dz), bf16 copy
@@ -209,6 +239,8 @@ static void free_scratch(MlpState &m) {
     fr(m.y);
     fr(m.dz);
     fr(m.dzb);
+    fr(m.part);
+    m.part = nullptr;
     m.x = nullptr;
     m.z1 = m.z2 = nullptr;
     m.h1 = m.h2 = nullptr;
@@ -242,6 +274,7 @@ hipError_t mlp_reserve(MlpState &m, size_t rows) {
     if ((e = hipMalloc(&m.y, rows * 4)) != hipSuccess) return e;
     if ((e = hipMalloc(&m.dz, rows * kMlpH * 4)) != hipSuccess) return e;
     if ((e = hipMalloc(&m.dzb, rows * kMlpH * 2)) != hipSuccess) return e;
+    if ((e = hipMalloc(&m.part, ((rows + kSumRows - 1) / kSumRows) * kMlpH * 4)) != hipSuccess) return e;
     m.cap = rows;
     return hipSuccess;
 }
@@ -293,9 +326,19 @@ hipError_t mlp_init_random(MlpState &m, uint64_t seed, hipStream_t s) {
 hipError_t mlp_features_items(MlpState &m, int width, const Item *items, const int *thr, int n, hipStream_t s) {
     dim3 grid((unsigned)((n + 3) / 4));
     if (width == 4)
-        hipLaunchKernelGGL((k_features<4>), grid, dim3(256), 0, s, items, thr, n, m.x);
+        hipLaunchKernelGGL((k_features<4>), grid, dim3(256), 0, s, items, thr, n, m.x, nullptr, nullptr, nullptr);
     else
-        hipLaunchKernelGGL((k_features<5>), grid, dim3(256), 0, s, items, thr, n, m.x);
+        hipLaunchKernelGGL((k_features<5>), grid, dim3(256), 0, s, items, thr, n, m.x, nullptr, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t mlp_features_gather(MlpState &m, int width, const Item *items, const int *thr, const float *tgt,
+                               const int *perm, int n, float *tout, hipStream_t s) {
+    dim3 grid((unsigned)((n + 3) / 4));
+    if (width == 4)
+        hipLaunchKernelGGL((k_features<4>), grid, dim3(256), 0, s, items, thr, n, m.x, perm, tgt, tout);
+    else
+        hipLaunchKernelGGL((k_features<5>), grid, dim3(256), 0, s, items, thr, n, m.x, perm, tgt, tout);
     return hipGetLastError();
 }
 
@@ -315,28 +358,38 @@ hipError_t mlp_forward(MlpState &m, int n, hipStream_t s) {
 }
 
 // Requires mlp_forward on the same rows.  Weights and activations are bf16,
-// biases fp32 (their gradients are fp32 sums of the fp32 dz).
-hipError_t mlp_backward(MlpState &m, const float *target, int n, hipStream_t s) {
+// biases fp32 (their gradients are fp32 sums of the fp32 dz).  The batch's
+// mean squared error is added to *loss_acc (loss_acc = m.loss after zeroing
+// it when null), so an epoch's losses can add up on the device.
+hipError_t mlp_backward(MlpState &m, const float *target, int n, hipStream_t s, float *loss_acc) {
     hipError_t e;
     float *dy = m.y + 0; // overwritten by dy after the loss (forward outputs consumed)
-    if ((e = hipMemsetAsync(m.loss, 0, 4, s)) != hipSuccess) return e;
+    if (!loss_acc) {
+        if ((e = hipMemsetAsync(m.loss, 0, 4, s)) != hipSuccess) return e;
+        loss_acc = m.loss;
+    }
     // dy overwrites y in place: each thread reads y[i] before writing dy[i]
-    hipLaunchKernelGGL(k_loss, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m.y, target, n, dy, m.loss);
-    hipLaunchKernelGGL(k_out_grad, dim3(1), dim3(kMlpH), 0, s, m.h2, m.z2, dy, m.wbf, m.grad, m.dz, m.dzb, n);
+    hipLaunchKernelGGL(k_loss, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m.y, target, n, dy, loss_acc);
+    const int nb = (n + kSumRows - 1) / kSumRows;
+    // gw3 = h2^T dy, gb3 = sum dy
+    hipLaunchKernelGGL(k_colsum_part_w, dim3((unsigned)nb), dim3(kMlpH), 0, s, m.h2, dy, n, m.part);
+    hipLaunchKernelGGL(k_colsum_fin, dim3(1), dim3(kMlpH), 0, s, m.part, nb, m.grad + kMlpOffW3);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, dy, n, m.grad + kMlpOffB3);
+    unsigned g = (unsigned)(((long)n * kMlpH + 255) / 256);
+    hipLaunchKernelGGL(k_out_dz, dim3(g), dim3(256), 0, s, m.z2, dy, m.wbf, m.dz, m.dzb, n);
     // gW2 = h1^T dz2 ; gb2
     if ((e = gemm(m.h1, 1, kMlpH, m.dzb, kMlpH, 1, m.grad + kMlpOffW2, kMlpH, kMlpH, kMlpH, n, 0, s)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(kMlpH), 0, s, m.dz, n, m.grad + kMlpOffB2);
+    if ((e = colsum(m.dz, n, m.part, m.grad + kMlpOffB2, s)) != hipSuccess) return e;
     // dh1 = dz2 W2^T  (into z2's buffer is not allowed: z1 needed; use dz)
     float *dh1 = m.dz; // dz2 (fp32) no longer needed after gW2/gb2; dzb still holds dz2 in bf16
     if ((e = gemm(m.dzb, kMlpH, 1, m.wbf + kMlpOffW2, 1, kMlpH, dh1, kMlpH, n, kMlpH, kMlpH, 0, s)) != hipSuccess)
         return e;
-    unsigned g = (unsigned)(((long)n * kMlpH + 255) / 256);
     hipLaunchKernelGGL(k_relu_grad, dim3(g), dim3(256), 0, s, dh1, m.z1, m.dzb, n);
     // gW1 = x^T dz1 ; gb1
     if ((e = gemm(m.x, 1, kMlpIn, m.dzb, kMlpH, 1, m.grad + kMlpOffW1, kMlpH, kMlpIn, kMlpH, n, 0, s)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(kMlpH), 0, s, dh1, n, m.grad + kMlpOffB1);
+    if ((e = colsum(dh1, n, m.part, m.grad + kMlpOffB1, s)) != hipSuccess) return e;
     return hipGetLastError();
 }
 

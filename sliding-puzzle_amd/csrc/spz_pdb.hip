@@ -85,6 +85,19 @@ __device__ __forceinline__ uint32_t pl_rank(const int (&s)[6]) {
     return rank;
 }
 
+// rank of canonical pattern t's goal placement (each tile on its own cell),
+// the host-side twin of pl_rank
+uint32_t pdb_goal_rank(int t) {
+    const uint8_t *pat = t == 0 ? kPatA : kPatB;
+    uint32_t used = 0, rank = 0;
+    for (int i = 0; i < 6; ++i) {
+        const uint32_t below = used & ((1u << pat[i]) - 1u);
+        rank = rank * (uint32_t)(25 - i) + (uint32_t)(pat[i] - __builtin_popcount(below));
+        used |= 1u << pat[i];
+    }
+    return rank;
+}
+
 __device__ __forceinline__ void pl_unrank(uint32_t rank, int (&s)[6]) {
     int dig[6];
 #pragma unroll

@@ -836,19 +836,80 @@ __device__ __forceinline__ int path_f(const Item &it) {
 //      decreasing order (counting sort over 64 half-log2 buckets): frontier
 //      order (uniform), last iteration's load (the reference's load-based
 //      static LB, src/15puzzle_block_parallel.cu:490-563), or the learned
-//      prediction -- longest expected subtree first,
-//   4. clears the per-frontier-node loads.
+//      prediction -- longest expected subtree first.  With re-splitting
+//      (resplit > 0, previous-load or learned policy), a node whose
+//      estimate exceeds resplit x the mean of the nodes that fit is queued
+//      as its children (two levels: grandchildren) instead, as the
+//      reference re-splits heavy roots into ceil(load / mean) pieces,
+//   4. clears the per-frontier-node loads (a split node's starts at the
+//      children the split counted).
 // An iteration with no frontier work is drained at once.  Lanes run no
 // kernel meanwhile, so the resolution needs no atomics on instance state.
-__device__ __forceinline__ int lb_bucket(const KParams &p, int inst, unsigned i, int it) {
-    float key = 0.f;
-    if (p.lb == 2) {
-        key = it > 0 ? log2f((float)p.load[(size_t)inst * p.capl + i] + 1.f) : 0.f;
-    } else if (p.lb == 1) {
-        key = p.pred[((size_t)inst * p.capl + i) * kPredK + (it < kPredK ? it : kPredK - 1)];
-    }
+
+// Estimated subtree size of frontier node i in iteration it: the previous
+// iteration's count below it (previous-load policy) or the learned
+// prediction of log2(1 + size) (learned policy); 0 for frontier order.
+__device__ __forceinline__ float lb_log2_est(const KParams &p, int inst, unsigned i, int it) {
+    if (p.lb == 2) return it > 0 ? log2f((float)p.load[(size_t)inst * p.capl + i] + 1.f) : 0.f;
+    if (p.lb == 1) return p.pred[((size_t)inst * p.capl + i) * kPredK + (it < kPredK ? it : kPredK - 1)];
+    return 0.f;
+}
+// dispatch bucket of a key in half-log2 steps (64 buckets, largest first)
+__device__ __forceinline__ int key_bucket(float key) {
     int b = (int)(key * 2.f);
     return b < 0 ? 0 : (b > 63 ? 63 : b);
+}
+// bucket of an item `level` split levels below a node of bucket b: each level
+// divides the estimate by about 3 (the children of a node), 3 half-log2 steps
+__device__ __forceinline__ int split_bucket(int b, int level) {
+    int c = b - 3 * level;
+    return c < 0 ? 0 : c;
+}
+
+// Re-split of a heavy frontier node (the reference's static load balancing,
+// src/15puzzle_block_parallel.cu:490-563 with divide_root_set :325-375): the
+// node is expanded here, its count of generated children added to the
+// iteration (the lane that would take it counts the same children,
+// src/24puzzle.cc:119-131), and its children whose f fits the threshold are
+// queued in its place -- or, at two levels, their children.  A goal node is
+// queued itself (its lane records the goal).  emit(pos, g, prev, level, d1,
+// d2) receives each item; returns the generated-children count.
+template <int W, int HEUR, class Emit>
+__device__ unsigned split_node(const PdbCtx &P, const Item &x, int L, int levels, Emit &&emit) {
+    Pos<W, HEUR> s;
+    pos_from_item<W, HEUR>(s, x.b, x.meta, P);
+    const int g = meta_g(x.meta), prev = meta_prev(x.meta);
+    unsigned gen_total = 0;
+    uint32_t gen = (s.br > 0 ? 1u : 0u) | (s.bc < W - 1 ? 2u : 0u) | (s.bc > 0 ? 4u : 0u) |
+                   (s.br < W - 1 ? 8u : 0u);
+    gen &= ~(prev < 4 ? (1u << (3 - prev)) : 0u);
+    gen_total += (unsigned)__builtin_popcount(gen);
+#pragma unroll 1
+    while (gen) {
+        const int d = __builtin_ctz(gen);
+        gen &= gen - 1;
+        Pos<W, HEUR> c = s;
+        do_move<W, HEUR>(c, d, P);
+        if (g + 1 + c.hs.h > L) continue;
+        if (levels < 2 || c.hs.h == 0) {
+            emit(c, g + 1, d, 1, d, 0);
+            continue;
+        }
+        uint32_t gen2 = (c.br > 0 ? 1u : 0u) | (c.bc < W - 1 ? 2u : 0u) | (c.bc > 0 ? 4u : 0u) |
+                        (c.br < W - 1 ? 8u : 0u);
+        gen2 &= ~(1u << (3 - d));
+        gen_total += (unsigned)__builtin_popcount(gen2);
+#pragma unroll 1
+        while (gen2) {
+            const int d2 = __builtin_ctz(gen2);
+            gen2 &= gen2 - 1;
+            Pos<W, HEUR> c2 = c;
+            do_move<W, HEUR>(c2, d2, P);
+            if (g + 2 + c2.hs.h > L) continue;
+            emit(c2, g + 2, d2, 2, d, d2);
+        }
+    }
+    return gen_total;
 }
 
 // thread 0 of k_restart: resolve drained iterations; returns the index of an
@@ -886,17 +947,31 @@ __device__ int resolve_instance(const KParams &p, int inst) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_restart(KParams p) {
+template <int W, int HEUR>
+__global__ void __launch_bounds__(256) k_restart(KParams p, const PdbGeo *geo_g) {
     __shared__ unsigned long long s_cnt;
     __shared__ unsigned s_bucket[64];
     __shared__ unsigned long long s_base;
     __shared__ int s_j, s_ring;
+    __shared__ float s_est;
+    __shared__ unsigned s_nfit;
+    __shared__ PdbGeo geo_s;
     const unsigned n_list = p.ctl[0];
     if (blockIdx.x >= n_list) return;
     const int inst = p.restart_list[blockIdx.x];
     const unsigned nf = min(p.nfront[inst], (unsigned)p.capl);
     const Item *F = p.frontier + (size_t)inst * p.capl;
     unsigned long long *load = p.load + (size_t)inst * p.capl;
+    // heavy-node re-split: only under a policy with estimates
+    const bool resplit = p.resplit > 0 && p.lb != 0;
+    PdbCtx P{&geo_s, p.pdb};
+    if constexpr (HEUR == 1) {
+        if (resplit) {
+            const uint32_t *src = (const uint32_t *)geo_g;
+            uint32_t *dst = (uint32_t *)&geo_s;
+            for (int i = threadIdx.x; i < (int)(sizeof(PdbGeo) / 4); i += blockDim.x) dst[i] = src[i];
+        }
+    }
 
     for (;;) {
         if (threadIdx.x == 0) {
@@ -924,17 +999,57 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
                 }
             }
         }
-        if (threadIdx.x == 0) s_cnt = 0;
+        if (threadIdx.x == 0) {
+            s_cnt = 0;
+            s_est = 0.f;
+            s_nfit = 0;
+        }
         for (int b = threadIdx.x; b < 64; b += blockDim.x) s_bucket[b] = 0;
         __syncthreads();
-        // 2. top tree: sum of gen over path-f <= L
+        // 2. top tree: sum of gen over path-f <= L; with re-splitting, the
+        // mean estimated subtree size of the frontier nodes that fit
         unsigned long long c = 0;
         for (int f = threadIdx.x; f < kFBins; f += blockDim.x)
             if (f <= L) c += p.topgen[(size_t)inst * kFBins + f];
         atomicAdd(&s_cnt, c);
-        // 3. bucket sizes of the frontier nodes that fit
-        for (unsigned i = threadIdx.x; i < nf; i += blockDim.x)
-            if (path_f(F[i]) <= L) atomicAdd(&s_bucket[lb_bucket(p, inst, i, it)], 1u);
+        if (resplit) {
+            float e = 0.f;
+            unsigned k = 0;
+            for (unsigned i = threadIdx.x; i < nf; i += blockDim.x)
+                if (path_f(F[i]) <= L) {
+                    e += exp2f(lb_log2_est(p, inst, i, it)) - 1.f;
+                    ++k;
+                }
+            atomicAdd(&s_est, e);
+            atomicAdd(&s_nfit, k);
+        }
+        __syncthreads();
+        // a node is split one level when its estimate exceeds resplit x the
+        // mean, two levels beyond 8 x that
+        const float cut1 = resplit && s_nfit ? (float)p.resplit * s_est / (float)s_nfit : 3.4e38f;
+        const float cut2 = 8.f * cut1;
+        auto levels_of = [&](unsigned i, float key) -> int {
+            if (!resplit) return 0;
+            const float e = exp2f(key) - 1.f;
+            return e > cut2 ? 2 : (e > cut1 ? 1 : 0);
+        };
+        // 3. bucket sizes of the items (frontier nodes that fit, or the items
+        // a split node becomes); split nodes' generated children are counted
+        for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) {
+            const Item x = F[i];
+            if (path_f(x) > L) continue;
+            const float key = lb_log2_est(p, inst, i, it);
+            const int b = key_bucket(key);
+            const int lv = levels_of(i, key);
+            if (lv == 0 || meta_h(x.meta) == 0) {
+                atomicAdd(&s_bucket[b], 1u);
+                continue;
+            }
+            const unsigned gn = split_node<W, HEUR>(P, x, L, lv, [&](const Pos<W, HEUR> &, int, int, int level, int, int) {
+                atomicAdd(&s_bucket[split_bucket(b, level)], 1u);
+            });
+            atomicAdd(&s_cnt, (unsigned long long)gn);
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
             unsigned off = 0;
@@ -953,19 +1068,49 @@ __global__ void __launch_bounds__(256) k_restart(KParams p) {
             p.dead[isl] = 0;
         }
         __syncthreads();
+        // 4. the items to the queue; the per-node loads of the new iteration
+        // start from zero, or from the children a split counted
         for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) {
             Item x = F[i];
+            unsigned long long ld = 0;
             if (path_f(x) <= L) {
-                unsigned k = atomicAdd(&s_bucket[lb_bucket(p, inst, i, it)], 1u);
-                x.inst = (uint32_t)isl;
-                x.x1 = i;
-                const size_t slot = q_slot(p, s_ring, s_base + k);
-                p.q[slot] = x;
-                if (p.qpath) path_from_u64(p.fpath[(size_t)inst * p.capl + i], p.qpath, (uint32_t)slot * kPathWords);
+                const float key = lb_log2_est(p, inst, i, it);
+                const int b = key_bucket(key);
+                const int lv = levels_of(i, key);
+                if (lv == 0 || meta_h(x.meta) == 0) {
+                    unsigned k = atomicAdd(&s_bucket[b], 1u);
+                    x.inst = (uint32_t)isl;
+                    x.x1 = i;
+                    const size_t slot = q_slot(p, s_ring, s_base + k);
+                    p.q[slot] = x;
+                    if (p.qpath) path_from_u64(p.fpath[(size_t)inst * p.capl + i], p.qpath, (uint32_t)slot * kPathWords);
+                } else {
+                    const int g0 = meta_g(x.meta);
+                    unsigned made = 0;
+                    ld = split_node<W, HEUR>(P, x, L, lv, [&](const Pos<W, HEUR> &cp, int g, int prev, int level, int d1, int d2) {
+                        ++made;
+                        unsigned k = atomicAdd(&s_bucket[split_bucket(b, level)], 1u);
+                        Item ci;
+                        cp.bd.pack(ci.b);
+                        ci.meta = meta_pack(cp.br * W + cp.bc, g, cp.hs.h, prev);
+                        ci.inst = (uint32_t)isl;
+                        ci.x0 = 0;
+                        ci.x1 = i;
+                        const size_t slot = q_slot(p, s_ring, s_base + k);
+                        p.q[slot] = ci;
+                        if (p.qpath) {
+                            const uint32_t qo = (uint32_t)slot * kPathWords;
+                            path_from_u64(p.fpath[(size_t)inst * p.capl + i], p.qpath, qo);
+                            path_put(p.qpath, qo, 1, g0, d1);
+                            if (level == 2) path_put(p.qpath, qo, 1, g0 + 1, d2);
+                        }
+                    });
+                    atomicAdd(&p.ctl[14], 1u);
+                    atomicAdd(&p.ctl[15], made);
+                }
             }
+            load[i] = ld;
         }
-        // 4. loads of the new iteration start from zero
-        for (unsigned i = threadIdx.x; i < nf; i += blockDim.x) load[i] = 0;
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -1218,8 +1363,13 @@ const void *round_kernel_ptr(int width, int heur) {
     return (const void *)&k_round<5, 1>;
 }
 
-hipError_t launch_restart(const KParams &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_restart, dim3(p.n_inst), dim3(256), 0, s, p);
+hipError_t launch_restart(int width, int heur, const KParams &p, const PdbGeo *geo, hipStream_t s) {
+    if (width == 4)
+        hipLaunchKernelGGL((k_restart<4, 0>), dim3(p.n_inst), dim3(256), 0, s, p, geo);
+    else if (heur == 0)
+        hipLaunchKernelGGL((k_restart<5, 0>), dim3(p.n_inst), dim3(256), 0, s, p, geo);
+    else
+        hipLaunchKernelGGL((k_restart<5, 1>), dim3(p.n_inst), dim3(256), 0, s, p, geo);
     return hipGetLastError();
 }
 

@@ -28,8 +28,13 @@ static void usage() {
                  "                         0 up, 1 right, 2 left, 3 down)\n"
                  "  --csv FILE             write per-instance seconds\n"
                  "  --pdb-cache FILE       read the PDB tables from FILE, or write them there\n"
+                 "  --pdb-tab DIR          load the reference's pat24.1256712.tab and\n"
+                 "                         pat24.34891314.tab from DIR instead of building them\n"
                  "  --export-tab DIR       write pat24.1256712.tab and pat24.34891314.tab\n"
-                 "  --lb uniform|prevload  dispatch order of frontier nodes\n"
+                 "  --lb uniform|prevload|learned  dispatch policy of frontier nodes (learned\n"
+                 "                         needs --mlp-params)\n"
+                 "  --mlp-params FILE      predictor parameters: spz_mlp_param_count() float32\n"
+                 "  --resplit K            re-split frontier nodes above K x the mean estimate\n"
                  "  --frontier N --wg N --lds-depth N --budget N --max-iters N\n"
                  "  --autotune             tune the launch configuration on the files first\n"
                  "  --no-lookahead         run each instance's iterations strictly one at a time\n"
@@ -58,7 +63,8 @@ int main(int argc, char **argv) {
     int heur = SPZ_H_MANHATTAN, mode = SPZ_MODE_ALL, lb = SPZ_LB_UNIFORM;
     int frontier = 2048, wg = 256, depth = 0, budget = 0, max_iters = 64, round_us = 0;
     bool batch = false, iters = false, autotune = false, path = false, lookahead = true, dedup = false;
-    std::string csv, cache, tabdir;
+    std::string csv, cache, tabdir, tabin, mlpfile;
+    int resplit = 0;
     std::vector<std::string> files;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -86,9 +92,24 @@ int main(int argc, char **argv) {
             cache = next();
         } else if (a == "--export-tab") {
             tabdir = next();
+        } else if (a == "--pdb-tab") {
+            tabin = next();
+        } else if (a == "--mlp-params") {
+            mlpfile = next();
+        } else if (a == "--resplit") {
+            resplit = std::atoi(next());
         } else if (a == "--lb") {
             std::string v = next();
-            lb = v == "prevload" ? SPZ_LB_PREVLOAD : SPZ_LB_UNIFORM;
+            if (v == "prevload")
+                lb = SPZ_LB_PREVLOAD;
+            else if (v == "learned")
+                lb = SPZ_LB_LEARNED;
+            else if (v == "uniform")
+                lb = SPZ_LB_UNIFORM;
+            else {
+                usage();
+                return 2;
+            }
         } else if (a == "--frontier") {
             frontier = std::atoi(next());
         } else if (a == "--wg") {
@@ -140,6 +161,11 @@ int main(int argc, char **argv) {
     cfg.lookahead = lookahead ? 1 : 0;
     cfg.round_time_us = round_us;
     cfg.dedup = dedup ? 1 : 0;
+    cfg.resplit = resplit;
+    if (lb == SPZ_LB_LEARNED && mlpfile.empty()) {
+        std::fprintf(stderr, "spz_solve: --lb learned needs --mlp-params FILE\n");
+        return 2;
+    }
     spz_ctx *ctx = nullptr;
     int rc = spz_create(&ctx, &cfg);
     if (rc != SPZ_OK) {
@@ -147,8 +173,31 @@ int main(int argc, char **argv) {
         spz_destroy(ctx);
         return 1;
     }
+    if (!mlpfile.empty()) {
+        std::vector<float> prm((size_t)spz_mlp_param_count());
+        FILE *f = std::fopen(mlpfile.c_str(), "rb");
+        const size_t got = f ? std::fread(prm.data(), sizeof(float), prm.size(), f) : 0;
+        if (f) std::fclose(f);
+        if (got != prm.size()) {
+            std::fprintf(stderr, "spz_solve: %s does not hold %zu float32 parameters\n", mlpfile.c_str(),
+                         prm.size());
+            spz_destroy(ctx);
+            return 1;
+        }
+        rc = spz_mlp_set_params(ctx, prm.data());
+        if (rc != SPZ_OK) {
+            std::fprintf(stderr, "spz_solve: %s\n", spz_last_error(ctx));
+            spz_destroy(ctx);
+            return 1;
+        }
+    }
     if (heur == SPZ_H_PDB666 || !tabdir.empty()) {
-        rc = spz_build_pdb(ctx, cache.empty() ? nullptr : cache.c_str());
+        if (!tabin.empty()) {
+            std::string a = tabin + "/pat24.1256712.tab", b = tabin + "/pat24.34891314.tab";
+            rc = spz_load_pdb_tab(ctx, a.c_str(), b.c_str());
+        } else {
+            rc = spz_build_pdb(ctx, cache.empty() ? nullptr : cache.c_str());
+        }
         if (rc == SPZ_OK && !tabdir.empty()) {
             std::string a = tabdir + "/pat24.1256712.tab", b = tabdir + "/pat24.34891314.tab";
             rc = spz_export_pdb_tab(ctx, a.c_str(), b.c_str());

@@ -28,10 +28,12 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libspz.so")
 H_MANHATTAN, H_PDB666 = 0, 1
 MODE_ALL, MODE_FIRST = 0, 1
 LB_UNIFORM, LB_LEARNED, LB_PREVLOAD = 0, 1, 2
+ST_SOLVED, ST_ITER_LIMIT = 0, 1
 MAX_ITERS = 128
 
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "no PDB", -4: "queue capacity",
-          -5: "I/O error", -6: "unsolvable", -7: "stalled", -8: "timeout"}
+          -5: "I/O error", -6: "unsolvable", -7: "stalled", -8: "timeout",
+          -9: "autotune result mismatch"}
 
 
 class SpzError(RuntimeError):
@@ -49,7 +51,7 @@ class Config(ctypes.Structure):
                 ("log_samples", ctypes.c_int32),
                 ("queue_items", ctypes.c_int64), ("record_paths", ctypes.c_int32),
                 ("lookahead", ctypes.c_int32), ("round_time_us", ctypes.c_int32),
-                ("dedup", ctypes.c_int32)]
+                ("dedup", ctypes.c_int32), ("resplit", ctypes.c_int32)]
 
 
 class _Result(ctypes.Structure):
@@ -72,7 +74,7 @@ class Result:
 _lib = None
 
 # every entry point declared in include/spz.h
-EXPORTS = ["spz_default_config", "spz_create", "spz_build_pdb", "spz_export_pdb_tab",
+EXPORTS = ["spz_default_config", "spz_create", "spz_build_pdb", "spz_load_pdb_tab", "spz_export_pdb_tab",
            "spz_get_pdb", "spz_solve_batch", "spz_eval_h", "spz_set_lb", "spz_last_stats",
            "spz_last_error", "spz_destroy", "spz_pdb_geometry", "spz_gen_walk",
            "spz_gen_random", "spz_gather_probe", "spz_mlp_param_count", "spz_mlp_init",
@@ -95,6 +97,7 @@ def lib() -> ctypes.CDLL:
     L.spz_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Config)]
     L.spz_build_pdb.argtypes = [vp, ctypes.c_char_p]
     L.spz_export_pdb_tab.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p]
+    L.spz_load_pdb_tab.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p]
     L.spz_get_pdb.argtypes = [vp, ctypes.c_int32, u8p]
     L.spz_solve_batch.argtypes = [vp, u8p, ctypes.c_int32, ctypes.c_int32,
                                   ctypes.POINTER(_Result), ctypes.POINTER(ctypes.c_uint64)]
@@ -187,7 +190,7 @@ class Solver:
                  lb_policy: int = LB_UNIFORM, device: int = 0, queue_items: int = 0,
                  time_kernels: bool = False, lds_stack_depth: int = 0, log_samples: int = 0,
                  use_graph: bool = True, record_paths: bool = False, lookahead: bool = True,
-                 round_time_us: int = 0, dedup: bool = False):
+                 round_time_us: int = 0, dedup: bool = False, resplit: int = 0):
         L = lib()
         cfg = Config()
         L.spz_default_config(ctypes.byref(cfg), width, heuristic)
@@ -208,6 +211,7 @@ class Solver:
         cfg.lookahead = 1 if lookahead else 0
         cfg.round_time_us = round_time_us
         cfg.dedup = 1 if dedup else 0
+        cfg.resplit = resplit
         self.cfg = cfg
         self.width = width
         self._ctx = ctypes.c_void_p()
@@ -244,6 +248,11 @@ class Solver:
     def build_pdb(self, cache_path: Optional[str] = None):
         self._check(lib().spz_build_pdb(self._ctx, cache_path.encode() if cache_path else None),
                     "spz_build_pdb")
+
+    def load_pdb_tab(self, path0: str, path1: str):
+        """Load the reference's pat24.1256712.tab and pat24.34891314.tab."""
+        self._check(lib().spz_load_pdb_tab(self._ctx, path0.encode(), path1.encode()),
+                    "spz_load_pdb_tab")
 
     def export_pdb_tab(self, path0: str, path1: str):
         self._check(lib().spz_export_pdb_tab(self._ctx, path0.encode(), path1.encode()),
@@ -404,7 +413,7 @@ class Solver:
         return res
 
     def stats(self) -> dict:
-        s = (ctypes.c_double * 24)()
+        s = (ctypes.c_double * 32)()
         self._check(lib().spz_last_stats(self._ctx, s), "spz_last_stats")
         return {"rounds": s[0], "solve_ms": s[1], "donated": s[2], "lanes": s[3],
                 "pdb_levels0": s[4], "pdb_levels1": s[5], "pdb_build_ms": s[6],
@@ -413,4 +422,4 @@ class Solver:
                 "setup_ms": s[13], "rounds_wall_ms": s[14], "graph": s[15],
                 "prof_idle": s[16], "prof_expand": s[17], "prof_pop": s[18], "prof_wave_steps": s[19],
                 "prof_sibling_moves": s[20], "prof_reads_first": s[21], "prof_reads_second": s[22],
-                "lookahead_starts": s[23]}
+                "lookahead_starts": s[23], "resplit_nodes": s[24], "resplit_items": s[25]}

@@ -13,12 +13,18 @@ Fixtures (all data: boards and expected outputs):
                           reference's src_horie/idas_cpu_expand.c and
                           idas_cpu_25_expand.c (built by oracle/ref.mk) on
                           seeded instances of the benchmark configurations
+  pdb_c4_walk160.json,    24-puzzle PDB configurations: lengths and totals of the
+  pdb_c5_walk240.json     reference's src/24puzzle_expand_with_pdb.cpp, per-iteration
+                          counts of the oracle (sums checked against them)
+  iterations_c3_mixed.json  the 15-puzzle batch configuration (walks and the first
+                          permutations), from src_horie/idas_cpu_expand.c
   pdb_tables.json         checksums of the two 6-tile tables, recorded only
                           after the reference's own PDB solver
                           (src/24puzzle_expand_with_pdb.cpp) run on them
                           reproduces benchmarks/pdb_expand_nodes.csv
 
 Usage: python tests/golden/make_golden.py [sets|iters15|iters24|c2|pdb|all]
+       python tests/golden/make_golden.py c3 | c4 | c5 I,J,...
 """
 from __future__ import annotations
 
@@ -199,8 +205,151 @@ def make_pdb():
         "tables": info})
 
 
+# ------------------------------------------- 24-puzzle PDB configurations ---
+def _tab_paths():
+    """The two tables in the reference's .tab layout, as make_pdb() writes
+    them (checked there against benchmarks/pdb_expand_nodes.csv 50/50)."""
+    d = os.path.join(REPO, "oracle", "_ref", "pdb", "blank0")
+    paths = (os.path.join(d, "pat24.1256712.tab"), os.path.join(d, "pat24.34891314.tab"))
+    if not all(os.path.exists(p) for p in paths):
+        raise SystemExit("PDB tables missing: run `make_golden.py pdb` first")
+    return paths
+
+
+def _ref_pdb_job(args):
+    """One board through the reference's src/24puzzle_expand_with_pdb.cpp:
+    (optimal length, all-solutions cumulative node count, its own seconds)."""
+    idx, board, paths = args
+    times = []
+    (ln, nodes), = R.yamamura("pdb_all", [board], paths, timeout=4 * 3600, times=times)
+    return idx, ln, nodes, times[0] if times else None
+
+
+def _orc_init(paths):
+    import numpy as np
+
+    O.pdb_set_tables(np.fromfile(paths[0], dtype=np.uint8), np.fromfile(paths[1], dtype=np.uint8))
+
+
+def _orc_pdb_job(args):
+    idx, board = args
+    opt, thr, cnt = O.ida(board, "pdb", "all")
+    return idx, opt, thr, cnt
+
+
+def _cache_path(name, what):
+    d = os.path.join(REPO, "oracle", "_ref", "cache")
+    os.makedirs(d, exist_ok=True)
+    return os.path.join(d, f"{name}.{what}.json")
+
+
+def _load_cache(name, what):
+    try:
+        with open(_cache_path(name, what)) as f:
+            return {int(k): v for k, v in json.load(f).items()}
+    except FileNotFoundError:
+        return {}
+
+
+def _save_cache(name, what, d):
+    with open(_cache_path(name, what), "w") as f:
+        json.dump(d, f)
+
+
+def pdb_fixture(name, config, boards, seeds, procs=8):
+    """Lengths and totals from the reference's PDB solver, per-iteration
+    counts from the oracle's restatement, for seeded 24-puzzle PDB boards.
+    The reference counts in a C int (src/24puzzle_expand_with_pdb.cpp), so
+    its printed total is the true total modulo 2^32, as a signed value: the
+    oracle's u64 sum must equal it modulo 2^32, and the fixture's "total" is
+    the oracle's sum.  Each solver's results are kept in oracle/_ref/cache
+    as they come in, so an interrupted run resumes."""
+    paths = _tab_paths()
+    boards = [[int(x) for x in b] for b in boards]
+    ref = _load_cache(name, "reference")
+    todo = [(i, b, paths) for i, b in enumerate(boards) if i not in ref]
+    with mp.Pool(procs) as pool:
+        for idx, ln, nodes, secs in pool.imap_unordered(_ref_pdb_job, todo):
+            ref[idx] = (ln, nodes, secs)
+            _save_cache(name, "reference", ref)
+            print(name, "reference", idx, ln, nodes, secs, flush=True)
+    orc = _load_cache(name, "oracle")
+    todo = [(i, b) for i, b in enumerate(boards) if i not in orc]
+    with mp.Pool(procs, initializer=_orc_init, initargs=(paths,)) as pool:
+        for idx, opt, thr, cnt in pool.imap_unordered(_orc_pdb_job, todo):
+            orc[idx] = (opt, thr, cnt)
+            _save_cache(name, "oracle", orc)
+            print(name, "oracle", idx, opt, sum(cnt), flush=True)
+    recs = []
+    for i, b in enumerate(boards):
+        ln, nodes, secs = ref[i]
+        opt, thr, cnt = orc[i]
+        if opt != ln or (sum(cnt) - nodes) % (1 << 32) != 0:
+            raise SystemExit(f"{name} board {i}: oracle {opt}/{sum(cnt)} vs reference {ln}/{nodes}")
+        recs.append({"idx": i, "seed": seeds[i], "board": b, "h0": thr[0], "opt": opt,
+                     "total": sum(cnt), "reference_printed_total": nodes, "thresholds": thr,
+                     "counts": cnt, "ref_seconds": secs})
+    dump(name, {"config": config,
+                "source": "lengths and totals: oracle/_ref/24puzzle_expand_with_pdb "
+                          "(src/24puzzle_expand_with_pdb.cpp) on the tables of make_pdb(); "
+                          "per-iteration counts: oracle/spz_oracle.c orc_ida (PDB, all "
+                          "solutions), whose sums equal the reference's totals modulo 2^32 "
+                          "(the reference counts in a C int: reference_printed_total)",
+                "note": "ref_seconds: the reference solver's own per-instance time, one core "
+                        "of an Intel Xeon @ 2.70 GHz, g++ -O3, 8 processes at once",
+                "instances": recs})
+
+
+def make_c4():
+    # C4: 64 non-backtracking 160-move walks on the 5x5 board, seed = 4e6 + i
+    seeds = [4_000_000 + i for i in range(64)]
+    pdb_fixture("pdb_c4_walk160.json", "c4", [O.walk(5, 160, s) for s in seeds], seeds)
+
+
+def make_c5(indices):
+    # C5: 64 non-backtracking 240-move walks, seed = 5e6 + i; the fixture
+    # holds the given instances (the smallest by node count: the whole set is
+    # about 3e12 nodes, days of CPU time)
+    seeds = [5_000_000 + i for i in indices]
+    pdb_fixture("pdb_c5_walk240.json", "c5", [O.walk(5, 240, s) for s in seeds], seeds)
+    fx = load("pdb_c5_walk240.json")
+    for rec, i in zip(fx["instances"], indices):
+        rec["idx"] = i
+    dump("pdb_c5_walk240.json", fx)
+
+
+def c3_walk_length(seed):
+    """Walk length of a C3 walk instance: 20 + (first splitmix64 output of
+    the seed, high 32 bits) * 61 >> 32, i.e. uniform in 20..80."""
+    m = 0xFFFFFFFFFFFFFFFF
+    z = (seed + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    z ^= z >> 31
+    return 20 + ((z >> 32) * 61 >> 32)
+
+
+def make_c3(n_random=64):
+    # C3: 4096 instances; 2048 walks (seed 3e6 + i, 20..80 moves) and 2048
+    # random solvable permutations (seed 3e6 + 2048 + i).  The fixture holds
+    # every walk and the first n_random permutations (each about 10 s of CPU).
+    boards = [O.walk(4, c3_walk_length(3_000_000 + i), 3_000_000 + i) for i in range(2048)]
+    boards += [O.random_solvable(4, 3_000_000 + 2048 + i) for i in range(n_random)]
+    horie_fixture("iterations_c3_mixed.json", "c3", boards, timeout=1800)
+    fx = load("iterations_c3_mixed.json")
+    for rec in fx["instances"]:
+        rec["seed"] = 3_000_000 + rec["idx"]
+    dump("iterations_c3_mixed.json", fx)
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what in ("c4",):
+        make_c4()
+    if what in ("c3",):
+        make_c3()
+    if what in ("c5",):
+        make_c5([int(x) for x in sys.argv[2].split(",")])
     if what in ("sets", "all"):
         make_sets()
     if what in ("pdb", "all"):

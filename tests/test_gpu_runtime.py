@@ -71,6 +71,29 @@ def test_autotuner_keeps_results(golden):
         assert all(b is None or a == b for a, b in zip(r.counts, rec["counts"]))
 
 
+@pytest.mark.parametrize("kw", [dict(mode=spz.MODE_FIRST), dict(dedup=True),
+                                dict(lb_policy=spz.LB_PREVLOAD)])
+def test_autotuner_other_modes(golden, kw):
+    """First-solution mode (the final count depends on the schedule), duplicate
+    pruning (counts depend on the split depth) and a policy with re-splitting:
+    the autotuner compares what each keeps and finishes."""
+    fx = golden("iterations_c1_walk40.json")["instances"][:24]
+    boards = [r["board"] for r in fx]
+    with spz.Solver(width=4, **kw) as s:
+        say("autotune", kw)
+        best, ms, tried = s.autotune(boards)
+        say("best", best, ms, tried)
+        assert tried >= 10 and ms > 0
+        if kw.get("dedup"):
+            assert best["frontier_target"] == 2048
+        res = s.solve(boards)
+    for rec, r in zip(fx, res):
+        assert r.opt_len == rec["opt"]
+        n = len(r.counts) - (1 if kw.get("mode") == spz.MODE_FIRST else 0)
+        if not kw.get("dedup"):
+            assert all(b is None or a == b for a, b in zip(r.counts[:n], rec["counts"][:n]))
+
+
 def test_cli_reference_format(tmp_path, golden, ref_results):
     """spz_solve prints "<path> <len> <nodes>" and, with --iters, the
     reference's per-iteration lines."""
