@@ -32,8 +32,13 @@ import numpy as np  # noqa: E402
 import spz  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # int32 lane-ops/s: CUs x SIMDs x lanes x clock
-VALU_ISSUE_PEAK = 256 * 2.4e9  # wave64 VALU instructions/s: 4 SIMD16 per CU, 4 cycles each
+# VALU peak of MI355X: the published 157.3 TFLOP/s FP32 vector rate is
+# 256 CUs x 2.4 GHz x 128 lanes x 2 (FMA), i.e. 128 lane-operations, two
+# wave64 instructions, per CU and cycle.  (The counters agree: the search
+# kernels' SQ_ACTIVE_INST_VALU quad-cycles exceed the SIMDs' elapsed
+# quad-cycles, so a wave64 instruction holds a SIMD for fewer than 4 cycles.)
+VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12  # int32 lane-ops/s
+VALU_ISSUE_PEAK = 256 * 2 * 2.4e9  # wave64 VALU instructions/s
 
 
 def instances(config: str, n: int | None = None):
@@ -188,7 +193,7 @@ def lpt_groups(items, weights, k):
     return [g for g in groups if g]
 
 
-def cpu_baseline(width, heur, boards, res, solver, budget_s=20.0):
+def cpu_baseline(width, heur, boards, res, solver, budget_s=15.0):
     """CPU baseline on the host cores, on a bounded sample of the workload:
     its instances in increasing order of their node count, until about
     budget_s seconds of work per core.  One solver process per core, the
@@ -250,12 +255,12 @@ def cpu_baseline(width, heur, boards, res, solver, budget_s=20.0):
             runs = list(ex.map(run, sorted(pick, key=lambda i: -nodes_of[i])))
         makespan = time.perf_counter() - t
         secs_sum, nodes = 0.0, 0
-        for i, (dt, work) in runs:
-            if work is None:  # solved in its first iteration: no count printed
+        for i, (dt, total, work) in runs:
+            if total is None:  # solved in its first iteration: no count printed
                 continue
-            assert work == res[i].total_nodes, i
+            assert total == res[i].total_nodes, i
             secs_sum += dt
-            nodes += work
+            nodes += work  # the nodes the process generated (its h0 tree twice)
         kind, who = "reference", "src_horie/idas_cpu_expand.c (oracle/_ref, gcc -O3)"
         used = cores
     else:
@@ -430,8 +435,8 @@ def main():
     kind = "pdb" if heur == spz.H_PDB666 else "manhattan"
     vi = profile_figure(pcfg, kind, "instructions", "valu_wave_instructions_per_node")
     if vi is not None:
-        # VALU issue: SQ_INSTS_VALU wave instructions against one wave64
-        # instruction per CU and cycle (4 SIMD16, 4 cycles each)
+        # VALU issue: SQ_INSTS_VALU wave instructions against two wave64
+        # instructions per CU and cycle
         rate = vi * nodes_per_launch / avg_launch_s
         roof["valu_issue"] = {
             "wave_instructions_per_node": vi, "achieved": rate / 1e9, "peak": VALU_ISSUE_PEAK / 1e9,

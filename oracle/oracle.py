@@ -49,6 +49,9 @@ def lib() -> ctypes.CDLL:
                               ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
                               ctypes.POINTER(ctypes.c_int),
                               ctypes.POINTER(ctypes.c_int)]
+        L.orc_ida_split.argtypes = [ctypes.c_int, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.orc_pdb_rank.argtypes = [u8p]
         L.orc_pdb_rank.restype = ctypes.c_uint32
         L.orc_pdb_unrank.argtypes = [ctypes.c_uint32, u8p]
@@ -116,6 +119,20 @@ def ida(board, heuristic: str = "manhattan", mode: str = "all",
     nit = ctypes.c_int(0)
     r = lib().orc_ida(width_of(arr), p, HEURISTICS[heuristic], MODES[mode],
                       max_iters, counts, thr, ctypes.byref(nit))
+    n = nit.value
+    return r, [thr[i] for i in range(n)], [counts[i] for i in range(n)]
+
+
+def ida_split(board, heuristic: str = "manhattan", target: int = 2048, max_depth: int = 24,
+              dedup: bool = False, max_iters: int = 64) -> Tuple[int, List[int], List[int]]:
+    """orc_ida's iterations over a root split of `target` frontier nodes, with
+    or without duplicate pruning of the top tree (all-solutions mode)."""
+    arr, p = _u8(board)
+    counts = (ctypes.c_uint64 * max_iters)()
+    thr = (ctypes.c_int * max_iters)()
+    nit = ctypes.c_int(0)
+    r = lib().orc_ida_split(width_of(arr), p, HEURISTICS[heuristic], target, max_depth,
+                            1 if dedup else 0, max_iters, counts, thr, ctypes.byref(nit))
     n = nit.value
     return r, [thr[i] for i in range(n)], [counts[i] for i in range(n)]
 

@@ -134,16 +134,21 @@ def yamamura(kind: str, boards, pdb_files=None, timeout: float = 3600, times=Non
     return yamamura_batch(binary, set_name, slots, boards, goal, pdb_files, timeout, times)
 
 
-def horie_timed(board: Sequence[int], timeout: float = 600) -> Tuple[float, Optional[int]]:
-    """Wall seconds (process start included, a few ms) and the number of
-    nodes Horie's solver generated: every printed iteration plus the first
-    one, whose count equals the h0+1 line (Manhattan parity).  None when it
-    solved the board in its first iteration (no count printed)."""
+def horie_timed(board: Sequence[int], timeout: float = 600) -> Tuple[float, Optional[int], Optional[int]]:
+    """Wall seconds (process start included, a few ms), the all-solutions
+    total of the thresholds h0, h0+2, ... and the nodes Horie's solver
+    actually generated.  Its schedule runs h0 and then h0+1, which with
+    Manhattan expands the same tree (src_horie/idas_cpu_expand.c:284-309),
+    and it prints 0 for the first: the h0+1 line is count(h0), so the total
+    is the sum of every line after the first, and the work is that plus
+    count(h0) once more.  (None, None) when it solved the board in its first
+    iteration (no count printed)."""
     import time
 
     t = time.perf_counter()
     lines = horie_iterations(board, timeout)
     dt = time.perf_counter() - t
     if len(lines) < 2:
-        return dt, None
-    return dt, sum(c for _, c in lines[1:]) + lines[1][1]
+        return dt, None, None
+    total = sum(c for _, c in lines[1:])
+    return dt, total, total + lines[1][1]

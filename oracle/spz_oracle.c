@@ -624,3 +624,195 @@ int orc_ida(int width, const uint8_t *board, int heuristic, int mode,
     }
     return -1;
 }
+
+/* ------------------------------------------------------------------------ */
+/* IDA* over a root split, optionally with duplicate pruning                 */
+/* ------------------------------------------------------------------------ */
+/* The counts of orc_ida computed the way a block-parallel solver splits the
+ * search (src/15puzzle_block_parallel.cu:150-187): the top of the tree is
+ * expanded breadth-first, level by level, until a level holds at least
+ * `target` nodes (or is at depth max_depth); that level is the frontier.
+ * Iteration L then counts the generated children of every top node whose
+ * path-max f (the largest f on its path from the root) is <= L, plus the
+ * depth-first count below every frontier node whose path-max f is <= L.
+ * Without pruning this equals orc_ida's count of every iteration.
+ * With dedup, before a level is expanded, a node is dropped when a kept
+ * node of the top tree (the root, a shallower node, or a copy in the same
+ * level with a smaller (path-max f, previous move)) holds its board with a
+ * path-max f no larger: the closed list of the reference's A* root
+ * distribution (src_horie/idas_smem.cu:1057-1149), whose counts no
+ * reference file holds. */
+typedef struct {
+    uint8_t b[25];
+    uint8_t g, h, F, prev; /* prev: 0..3, 4 = none */
+} tnode_t;
+
+static int g_n2; /* board size for the comparators (the oracle is single-threaded here) */
+
+static int cmp_board_f_prev(const void *x, const void *y)
+{
+    const tnode_t *a = (const tnode_t *)x, *b = (const tnode_t *)y;
+    int c = memcmp(a->b, b->b, (size_t)g_n2);
+    if (c)
+        return c;
+    if (a->F != b->F)
+        return (int)a->F - (int)b->F;
+    return (int)a->prev - (int)b->prev;
+}
+
+typedef struct {
+    uint8_t b[25];
+    uint8_t F;
+} seen_t;
+
+static int cmp_seen(const void *x, const void *y)
+{
+    return memcmp(((const seen_t *)x)->b, ((const seen_t *)y)->b, (size_t)g_n2);
+}
+
+static int node_h(int w, int heur, const uint8_t *b)
+{
+    if (heur == ORC_H_MANHATTAN)
+        return orc_manhattan(w, b);
+    uint8_t pos[25];
+    for (int i = 0; i < 25; ++i)
+        pos[b[i]] = (uint8_t)i;
+    return pdb_h_pos(pos);
+}
+
+/* prune a level in place against the sorted closed list; returns the new
+ * length (the closed list grows) */
+static size_t prune_level(tnode_t *lv, size_t n, seen_t **seen, size_t *ns, size_t *cap)
+{
+    qsort(lv, n, sizeof *lv, cmp_board_f_prev);
+    size_t out = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (i > 0 && memcmp(lv[i].b, lv[i - 1].b, (size_t)g_n2) == 0)
+            continue; /* not the board's best copy in this level */
+        seen_t key;
+        memcpy(key.b, lv[i].b, 25);
+        seen_t *hit = (seen_t *)bsearch(&key, *seen, *ns, sizeof key, cmp_seen);
+        if (hit) {
+            if (hit->F <= lv[i].F)
+                continue; /* a kept copy covers it */
+            hit->F = lv[i].F;
+        } else {
+            if (*ns == *cap) {
+                *cap = *cap ? 2 * *cap : 1024;
+                *seen = (seen_t *)realloc(*seen, *cap * sizeof(seen_t));
+            }
+            key.F = lv[i].F;
+            (*seen)[(*ns)++] = key;
+        }
+        lv[out++] = lv[i];
+    }
+    qsort(*seen, *ns, sizeof(seen_t), cmp_seen);
+    return out;
+}
+
+int orc_ida_split(int width, const uint8_t *board, int heuristic, int target, int max_depth,
+                  int dedup, int max_iters, uint64_t *counts, int *thresholds, int *n_iters)
+{
+    *n_iters = 0;
+    if (width < 2 || width > 5 || !orc_is_solvable(width, board) || target < 1 || max_depth < 0)
+        return -2;
+    if (heuristic == ORC_H_PDB && (width != 5 || !g_tab[0] || !g_tab[1]))
+        return -2;
+    const int n2 = width * width;
+    g_n2 = n2;
+    size_t cap = 16, n = 1;
+    tnode_t *lv = (tnode_t *)calloc(cap, sizeof(tnode_t));
+    memcpy(lv[0].b, board, (size_t)n2);
+    const int h0 = node_h(width, heuristic, board);
+    lv[0].g = 0;
+    lv[0].h = (uint8_t)h0;
+    lv[0].F = (uint8_t)h0;
+    lv[0].prev = 4;
+    seen_t *seen = NULL;
+    size_t ns = 0, scap = 0;
+    if (dedup) {
+        scap = 1024;
+        seen = (seen_t *)calloc(scap, sizeof(seen_t));
+        memcpy(seen[0].b, board, (size_t)n2);
+        seen[0].F = (uint8_t)h0;
+        ns = 1;
+    }
+    uint64_t topgen[256];
+    memset(topgen, 0, sizeof topgen);
+    int top_goal = 0x7fffffff;
+    for (int depth = 0; depth <= max_depth; ++depth) {
+        if (dedup && depth > 0)
+            n = prune_level(lv, n, &seen, &ns, &scap);
+        if (n >= (size_t)target || depth >= max_depth)
+            break;
+        size_t ncap = 4 * n, nn = 0;
+        tnode_t *nx = (tnode_t *)calloc(ncap, sizeof(tnode_t));
+        for (size_t i = 0; i < n; ++i) {
+            const tnode_t *u = &lv[i];
+            int blank = 0;
+            while (u->b[blank] != 0)
+                ++blank;
+            if (u->h == 0 && u->F < top_goal)
+                top_goal = u->F;
+            for (int d = 0; d < 4; ++d) {
+                if (u->prev < 4 && d == 3 - u->prev)
+                    continue;
+                int nb = neighbor(width, blank, d);
+                if (nb < 0)
+                    continue;
+                topgen[u->F] += 1;
+                tnode_t *c = &nx[nn++];
+                *c = *u;
+                c->b[blank] = u->b[nb];
+                c->b[nb] = 0;
+                c->g = (uint8_t)(u->g + 1);
+                c->h = (uint8_t)node_h(width, heuristic, c->b);
+                int f = c->g + c->h;
+                c->F = (uint8_t)(f > u->F ? f : u->F);
+                c->prev = (uint8_t)d;
+            }
+        }
+        free(lv);
+        lv = nx;
+        n = nn;
+    }
+    int result = -1;
+    for (int it = 0; it < max_iters; ++it) {
+        const int L = h0 + 2 * it;
+        uint64_t cnt = 0;
+        for (int f = 0; f < 256 && f <= L; ++f)
+            cnt += topgen[f];
+        int found = top_goal <= L ? top_goal : 0x7fffffff;
+        for (size_t i = 0; i < n; ++i) {
+            if (lv[i].F > L)
+                continue;
+            ida_t s;
+            memset(&s, 0, sizeof s);
+            s.w = width;
+            s.heur = heuristic;
+            s.mode = ORC_MODE_ALL;
+            s.limit = L;
+            for (int c = 0; c < n2; ++c) {
+                s.b[c] = lv[i].b[c];
+                s.pos[lv[i].b[c]] = (uint8_t)c;
+                if (lv[i].b[c] == 0)
+                    s.blank = c;
+            }
+            s.h = lv[i].h;
+            ida_dfs(&s, lv[i].g, lv[i].prev < 4 ? lv[i].prev : -1);
+            cnt += s.count;
+            if (s.found && s.found_depth < found)
+                found = s.found_depth;
+        }
+        counts[it] = cnt;
+        thresholds[it] = L;
+        *n_iters = it + 1;
+        if (found != 0x7fffffff) {
+            result = found;
+            break;
+        }
+    }
+    free(lv);
+    free(seen);
+    return result;
+}

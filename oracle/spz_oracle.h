@@ -68,6 +68,20 @@ uint64_t orc_rng_u64(uint64_t seed, int nth); /* nth output of the stream */
 int orc_ida(int width, const uint8_t *board, int heuristic, int mode,
             int max_iters, uint64_t *counts, int *thresholds, int *n_iters);
 
+/*
+ * The same iterations computed over a root split (top tree expanded level by
+ * level until a level holds >= target nodes or reaches max_depth; counts =
+ * top-tree generated children with path-max f <= L + depth-first counts below
+ * the frontier nodes with path-max f <= L), all-solutions mode.  dedup = 0
+ * equals orc_ida; dedup = 1 prunes duplicate boards from the top tree (a node
+ * whose board a kept node -- the root, a shallower node, or a same-level copy
+ * with smaller (path-max f, previous move) -- holds with a path-max f no larger
+ * is dropped before its level is expanded), the closed list of
+ * src_horie/idas_smem.cu:1057-1149.
+ */
+int orc_ida_split(int width, const uint8_t *board, int heuristic, int target, int max_depth,
+                  int dedup, int max_iters, uint64_t *counts, int *thresholds, int *n_iters);
+
 /* --- PDB (24-puzzle, 6-6-6-6) ------------------------------------------ */
 /* rank of 6 distinct cells in the reference's nested-loop file order */
 uint32_t orc_pdb_rank(const uint8_t cells[6]);

@@ -1,9 +1,11 @@
 """Duplicate pruning in the root split (config dedup): a top-tree node whose
-board an earlier kept node of its instance holds with a path-max f no larger
-is dropped.  Node counts then differ from the reference's by design, so these
-are properties, not parity: optimal lengths equal the golden ones, every
-iteration runs with the same thresholds, every per-iteration count is at most
-the exact one, and recorded solution paths stay optimal."""
+board a kept node of its instance holds with a path-max f no larger is
+dropped.  Node counts then differ from the reference's by design: the pruned
+counts are checked bit-exact against the oracle's restatement of the pruned
+split (orc_ida_split with dedup, whose unpruned form equals the reference-
+pinned orc_ida), and through properties: optimal lengths equal the golden
+ones, every iteration runs with the same thresholds, every per-iteration
+count is at most the exact one, and recorded solution paths stay optimal."""
 import sys
 
 import pytest
@@ -88,3 +90,22 @@ def test_dedup_is_deterministic():
     a, _ = solve(boards, width=4, frontier_target=4096, dedup=True)
     b, _ = solve(boards, width=4, frontier_target=4096, dedup=True)
     assert [(r.opt_len, r.counts) for r in a] == [(r.opt_len, r.counts) for r in b]
+
+
+@pytest.mark.parametrize("target", [64, 4096])
+def test_dedup_counts_match_oracle(golden, target):
+    """Every per-iteration count of the pruned split against the oracle."""
+    fx = golden("iterations_korf100.json")["instances"][:6]
+    boards = [r["board"] for r in fx] + [O.walk(4, 50, 5_800_000 + i) for i in range(14)]
+    dd, _ = solve(boards, width=4, frontier_target=target, dedup=True)
+    for b, r in zip(boards, dd):
+        opt, thr, cnt = O.ida_split(b, target=target, dedup=True)
+        assert (r.opt_len, r.thresholds, r.counts) == (opt, thr, cnt)
+
+
+def test_dedup_24_puzzle_manhattan_matches_oracle():
+    boards = [O.walk(5, 60, 5_900_000 + i) for i in range(8)]
+    dd, _ = solve(boards, width=5, frontier_target=1024, dedup=True)
+    for b, r in zip(boards, dd):
+        opt, thr, cnt = O.ida_split(b, target=1024, dedup=True)
+        assert (r.opt_len, r.thresholds, r.counts) == (opt, thr, cnt)

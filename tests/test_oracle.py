@@ -377,3 +377,24 @@ def test_pdb_first_solution_counts(pdb_tables, ref_sets, ref_results):
             continue
         r, thr, c = O.ida(ref_sets["yama24_50_med"][i], heuristic="pdb", mode="first")
         assert (r, sum(c)) == (row["len"], row["nodes"]), i
+
+
+def test_split_restatement_equals_plain_ida():
+    """orc_ida_split without pruning is orc_ida's count regrouped (top-tree
+    children with path-max f <= L plus the frontier nodes' subtrees): equal
+    for every frontier size, 15- and 24-puzzle; with pruning the lengths and
+    thresholds are kept and no count rises."""
+    boards = [O.walk(4, 40, 1_000_000 + i) for i in range(12)]
+    boards += [O.walk(4, 60, 2_100_000 + i) for i in range(3)]
+    boards += [O.walk(5, 40, 4_100_000 + i) for i in range(4)]
+    for b in boards:
+        exact = O.ida(b, max_iters=20)
+        if exact[0] < 0:
+            continue
+        for target in (1, 7, 64, 777, 5000):
+            assert O.ida_split(b, target=target, max_iters=20) == exact
+            opt, thr, cnt = O.ida_split(b, target=target, dedup=True, max_iters=20)
+            assert (opt, thr) == exact[:2]
+            assert all(x <= y for x, y in zip(cnt, exact[2]))
+        # a shallow depth cap freezes the split early: same counts
+        assert O.ida_split(b, target=5000, max_depth=3, max_iters=20) == exact
