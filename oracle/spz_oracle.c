@@ -685,13 +685,16 @@ static int node_h(int w, int heur, const uint8_t *b)
 static size_t prune_level(tnode_t *lv, size_t n, seen_t **seen, size_t *ns, size_t *cap)
 {
     qsort(lv, n, sizeof *lv, cmp_board_f_prev);
+    /* the list is sorted up to here; this level's new boards are appended
+     * after it (each board occurs once in the level) and sorted in at the end */
+    const size_t sorted = *ns;
     size_t out = 0;
     for (size_t i = 0; i < n; ++i) {
         if (i > 0 && memcmp(lv[i].b, lv[i - 1].b, (size_t)g_n2) == 0)
             continue; /* not the board's best copy in this level */
         seen_t key;
         memcpy(key.b, lv[i].b, 25);
-        seen_t *hit = (seen_t *)bsearch(&key, *seen, *ns, sizeof key, cmp_seen);
+        seen_t *hit = (seen_t *)bsearch(&key, *seen, sorted, sizeof key, cmp_seen);
         if (hit) {
             if (hit->F <= lv[i].F)
                 continue; /* a kept copy covers it */

@@ -133,7 +133,7 @@ def test_c4_full_batch(pdb24, golden, lb, resplit):
         pdb24.set_lb(spz.LB_PREVLOAD if lb == "prevload" else spz.LB_UNIFORM)
     say("c4: 64 instances,", lb, "resplit", resplit)
     res = pdb24.solve(boards)
-    if resplit:
+    if resplit and lb == "prevload":  # random parameters predict nearly equal sizes
         assert pdb24.stats()["resplit_nodes"] > 0
     pdb24.set_lb(spz.LB_UNIFORM)
     pdb24.reconfigure(resplit=0)
