@@ -203,6 +203,16 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n,
  * computed on the device; h must hold n int32. */
 int spz_eval_h(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h);
 
+/* spz_eval_h that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, allocation-free harnesses).  The boards and
+ * values pass through a static device array of the library (1.9 MB per GPU,
+ * held until the process ends): at most 65536 boards per call (SPZ_E_ARG
+ * beyond).  Every call on a GPU shares that array, so calls that overlap on
+ * one GPU (other threads, other contexts) race; serialise them. */
+#define SPZ_NOALLOC_MAX_BOARDS 65536
+int spz_eval_h_noalloc(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h);
+
 /* Change the dispatch policy (SPZ_LB_*) between solves.  The policy only
  * orders the frontier nodes an iteration queues; results never depend on
  * it.  Replaces the host re-split of heavy roots between iterations

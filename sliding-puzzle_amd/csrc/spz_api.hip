@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -30,6 +31,8 @@ hipError_t launch_split_level(int width, int heur, const KParams &p, const PdbGe
 hipError_t launch_eval_h(int width, int heur, const KParams &p, const PdbGeo *geo, Item *items, int n,
                          hipStream_t s);
 hipError_t launch_scatter_roots(const Item *roots, Item *lvl, int capl, int n, hipStream_t s);
+hipError_t launch_eval_h_bytes(int width, int heur, const uint8_t *bytes, int n, const PdbGeo *geo,
+                               const uint8_t *pdb, int *h, hipStream_t s);
 hipError_t pdb_bfs_build(int pat, uint8_t *compact, uint32_t *work, unsigned long long *d_cnt, hipStream_t s,
                          int *levels_out);
 hipError_t pdb_spread(const uint8_t *compact, uint8_t *direct, hipStream_t s);
@@ -94,6 +97,33 @@ struct HBuf {
 
 } // namespace
 
+// Scratch of spz_eval_h_noalloc: the boards (kNoAllocMaxBoards x 25 bytes),
+// then their heuristic values (kNoAllocMaxBoards int32): 1.9 MB on every GPU
+// the library runs on, held until the process ends.  Host code never uses
+// the array's name as an address: spz_create looks up each device's own
+// copy once (scratch_for_device) and keeps that pointer.
+constexpr int kNoAllocMaxBoards = 65536;
+constexpr size_t kNoAllocHOff = (size_t)kNoAllocMaxBoards * 25;
+static __device__ __align__(256) uint8_t scratch[kNoAllocHOff + (size_t)kNoAllocMaxBoards * 4];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// the device's copy of `scratch`; the device must be current
+static hipError_t scratch_for_device(int dev, void **out) {
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 struct spz_ctx {
     spz_config cfg;
     hipStream_t stream = nullptr;
@@ -131,6 +161,7 @@ struct spz_ctx {
     MlpState mlp;
     bool mlp_alloced = false;
     HBuf dd_items, dd_paths; // staging of the duplicate-pruning pass
+    uint8_t *scratch_d = nullptr; // this device's copy of `scratch` (spz_eval_h_noalloc)
 };
 
 static int fail(spz_ctx *c, int code, const char *fmt, ...) {
@@ -316,6 +347,15 @@ int spz_create(spz_ctx **out, const spz_config *cfg) {
         if (e != hipSuccess) {
             rc = fail(ctx, SPZ_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
             break;
+        }
+        {
+            void *sp = nullptr;
+            e = scratch_for_device(cfg->device, &sp);
+            if (e != hipSuccess) {
+                rc = fail(ctx, SPZ_E_HIP, "scratch address: %s", hipGetErrorString(e));
+                break;
+            }
+            ctx->scratch_d = (uint8_t *)sp;
         }
         pdb_geo_build(&ctx->geo_h);
         if ((e = ctx->geo_d.ensure(sizeof(PdbGeo))) != hipSuccess ||
@@ -784,6 +824,31 @@ int spz_eval_h(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h) {
     d.release();
     if (e != hipSuccess) return fail(ctx, SPZ_E_HIP, "eval_h: %s", hipGetErrorString(e));
     for (int i = 0; i < n; ++i) h[i] = meta_h(items[i].meta);
+    return SPZ_OK;
+}
+
+// spz_eval_h without any allocation: boards and values pass through the
+// device's `scratch` array (every call on a GPU shares it; calls that
+// overlap on one GPU race), the kernel reads the tables the context holds.
+int spz_eval_h_noalloc(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h) {
+    if (!ctx || !boards || !h || n < 0) return SPZ_E_ARG;
+    if (n == 0) return SPZ_OK;
+    if (n > kNoAllocMaxBoards)
+        return fail(ctx, SPZ_E_ARG, "spz_eval_h_noalloc takes at most %d boards (got %d)", kNoAllocMaxBoards, n);
+    if (!ctx->scratch_d) return fail(ctx, SPZ_E_HIP, "no scratch on this device");
+    const int w = ctx->cfg.width, heur = ctx->cfg.heuristic, n2 = w * w;
+    if (heur == SPZ_H_PDB666 && !ctx->have_pdb) return fail(ctx, SPZ_E_NOPDB, "call spz_build_pdb first");
+    for (int i = 0; i < n; ++i)
+        if (!is_solvable(w, boards + (size_t)i * n2))
+            return fail(ctx, SPZ_E_ARG, "board %d is not a solvable permutation", i);
+    HIPCHK(hipSetDevice(ctx->cfg.device));
+    hipStream_t s = ctx->stream;
+    int *hd = (int *)(ctx->scratch_d + kNoAllocHOff);
+    HIPCHK(hipMemcpyAsync(ctx->scratch_d, boards, (size_t)n * n2, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_eval_h_bytes(w, heur, ctx->scratch_d, n, ctx->geo_d.as<PdbGeo>(), ctx->pdb_direct.as<uint8_t>(),
+                               hd, s));
+    HIPCHK(hipMemcpyAsync(h, hd, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     return SPZ_OK;
 }
 

@@ -1319,6 +1319,60 @@ __global__ void k_eval_h(KParams p, const PdbGeo *geo_g, Item *items, int n) {
     items[i] = it;
 }
 
+// heuristic of n boards given as bytes (row-major, W*W per board) into
+// h[n]; the allocation-free evaluation entry point's kernel
+template <int W, int HEUR>
+__global__ void k_eval_h_bytes(const uint8_t *bytes, int n, const PdbGeo *geo_g, const uint8_t *pdb, int *h) {
+    __shared__ PdbGeo geo_s;
+    PdbCtx P{&geo_s, pdb};
+    if constexpr (HEUR == 1) {
+        const uint32_t *src = (const uint32_t *)geo_g;
+        uint32_t *dst = (uint32_t *)&geo_s;
+        for (int i = threadIdx.x; i < (int)(sizeof(PdbGeo) / 4); i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t *b = bytes + (size_t)i * (W * W);
+    Board<W> bd;
+    if constexpr (W == 4) {
+        bd.b = 0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) bd.b |= (uint64_t)b[c] << (4 * c);
+    } else {
+        bd.lo = bd.hi = 0;
+        bd.top = b[24];
+#pragma unroll
+        for (int c = 0; c < 12; ++c) bd.lo |= (uint64_t)b[c] << (5 * c);
+#pragma unroll
+        for (int c = 12; c < 24; ++c) bd.hi |= (uint64_t)b[c] << (5 * (c - 12));
+    }
+    int v = 0;
+    if constexpr (HEUR == 0) {
+        for (int c = 0; c < W * W; ++c) {
+            const int t = bd.tile(c);
+            if (t) v += abs(c / W - trow<W>(t)) + abs(c % W - tcol<W>(t));
+        }
+    } else {
+        HState<1> hs;
+        pdb_init(P, hs, bd);
+        v = hs.h;
+    }
+    h[i] = v;
+}
+
+hipError_t launch_eval_h_bytes(int width, int heur, const uint8_t *bytes, int n, const PdbGeo *geo,
+                               const uint8_t *pdb, int *h, hipStream_t s) {
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (width == 4)
+        hipLaunchKernelGGL((k_eval_h_bytes<4, 0>), grid, dim3(256), 0, s, bytes, n, geo, pdb, h);
+    else if (heur == 0)
+        hipLaunchKernelGGL((k_eval_h_bytes<5, 0>), grid, dim3(256), 0, s, bytes, n, geo, pdb, h);
+    else
+        hipLaunchKernelGGL((k_eval_h_bytes<5, 1>), grid, dim3(256), 0, s, bytes, n, geo, pdb, h);
+    return hipGetLastError();
+}
+
 // roots -> slot 0 of each instance's level-0 region
 __global__ void k_scatter_roots(const Item *roots, Item *lvl, int capl, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

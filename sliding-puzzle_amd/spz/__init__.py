@@ -29,6 +29,7 @@ H_MANHATTAN, H_PDB666 = 0, 1
 MODE_ALL, MODE_FIRST = 0, 1
 LB_UNIFORM, LB_LEARNED, LB_PREVLOAD = 0, 1, 2
 ST_SOLVED, ST_ITER_LIMIT = 0, 1
+NOALLOC_MAX_BOARDS = 65536  # SPZ_NOALLOC_MAX_BOARDS
 MAX_ITERS = 128
 
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "no PDB", -4: "queue capacity",
@@ -80,7 +81,7 @@ EXPORTS = ["spz_default_config", "spz_create", "spz_build_pdb", "spz_load_pdb_ta
            "spz_gen_random", "spz_gather_probe", "spz_mlp_param_count", "spz_mlp_init",
            "spz_mlp_set_params", "spz_mlp_get_params", "spz_mlp_forward", "spz_mlp_grad",
            "spz_mlp_train", "spz_get_samples", "spz_reconfigure", "spz_get_config",
-           "spz_autotune", "spz_mlp_predict", "spz_get_paths"]
+           "spz_autotune", "spz_mlp_predict", "spz_get_paths", "spz_eval_h_noalloc"]
 
 
 def lib() -> ctypes.CDLL:
@@ -102,6 +103,7 @@ def lib() -> ctypes.CDLL:
     L.spz_solve_batch.argtypes = [vp, u8p, ctypes.c_int32, ctypes.c_int32,
                                   ctypes.POINTER(_Result), ctypes.POINTER(ctypes.c_uint64)]
     L.spz_eval_h.argtypes = [vp, u8p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    L.spz_eval_h_noalloc.argtypes = [vp, u8p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
     L.spz_set_lb.argtypes = [vp, ctypes.c_int32]
     L.spz_last_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.spz_last_error.argtypes = [vp]
@@ -392,6 +394,16 @@ class Solver:
         self._check(lib().spz_eval_h(self._ctx, _u8p(b), len(b),
                                      h.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
                     "spz_eval_h")
+        return h
+
+    def eval_h_noalloc(self, boards) -> np.ndarray:
+        """eval_h through the allocation-free entry point (at most 65536
+        boards; calls on one GPU share its scratch and must not overlap)."""
+        b = np.ascontiguousarray(np.asarray(boards, dtype=np.uint8).reshape(-1, self.width ** 2))
+        h = np.zeros(len(b), dtype=np.int32)
+        self._check(lib().spz_eval_h_noalloc(self._ctx, _u8p(b), len(b),
+                                             h.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+                    "spz_eval_h_noalloc")
         return h
 
     def solve(self, boards, max_iters: int = 64) -> List[Result]:
