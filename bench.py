@@ -5,8 +5,12 @@
 A step is one solve of the configuration's whole batch of synthetic
 instances (seeded generators; seed = config_id * 10^6 + instance index),
 inputs already on the host in packed form; the per-step result (optimal
-lengths and node counts) is checked to be identical across steps.  Prints
-one JSON line.  Configurations:
+lengths and node counts) is checked to be identical across steps.  The
+timed steps replay the search rounds from the solver's captured hipGraph;
+a separate step with every round launch timed by HIP events gives the
+kernel's average duration (the roofline).  The default policy is the
+learned predictor with re-splitting of nodes predicted above 2 x the mean.
+Prints one JSON line.  Configurations:
   c1  15-puzzle Manhattan, 100 walks of 40 moves
   c2  15-puzzle Manhattan, 100 random solvable permutations, one instance
       per launch, root split into 2048 frontier nodes
@@ -327,7 +331,7 @@ def main():
     ap.add_argument("--wg", type=int, default=256)
     ap.add_argument("--lb", default="learned", choices=list(POLICIES),
                     help="dispatch policy of the timed solver (default: the learned predictor)")
-    ap.add_argument("--resplit", type=int, default=0,
+    ap.add_argument("--resplit", type=int, default=2,
                     help="re-split frontier nodes whose estimate exceeds K x the mean (0: off)")
     ap.add_argument("--no-ablation", action="store_true",
                     help="skip the one-step comparison solves under the other policies")
