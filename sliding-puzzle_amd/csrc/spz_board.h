@@ -288,30 +288,106 @@ template <class BD> __device__ __forceinline__ void pdb_init(const PdbCtx &P, HS
     s.h = s.A > s.B ? s.A : s.B;
 }
 
-// Lane position: board + blank (cell, row, col) + heuristic state.
+// Lane position: board + blank + heuristic state.  The blank is kept as its
+// row and column, or (15-puzzle) as its cell, from which row and column are
+// a shift and a mask.
 template <int W, int HEUR, class BD = Board<W>> struct Pos {
     BD bd;
     int br, bc;
     HState<HEUR> hs;
+    __device__ __forceinline__ int cell() const { return br * W + bc; }
+    __device__ __forceinline__ int row() const { return br; }
+    __device__ __forceinline__ int col() const { return bc; }
+    __device__ __forceinline__ void set_cell(int c) {
+        br = c / W;
+        bc = c - br * W;
+    }
+    __device__ __forceinline__ void step(int d) {
+        br += dir_dr(d);
+        bc += dir_dc(d);
+    }
 };
+template <int HEUR, class BD> struct Pos<4, HEUR, BD> {
+    BD bd;
+    int bl;
+    HState<HEUR> hs;
+    __device__ __forceinline__ int cell() const { return bl; }
+    __device__ __forceinline__ int row() const { return bl >> 2; }
+    __device__ __forceinline__ int col() const { return bl & 3; }
+    __device__ __forceinline__ void set_cell(int c) { bl = c; }
+    __device__ __forceinline__ void step(int d) { bl += dir_cell<4>(d); }
+};
+
+// moves the blank can make from cell c (bit d: direction d stays on the board)
+template <int W> __device__ __forceinline__ uint32_t legal_moves(int r, int c) {
+    return (r > 0 ? 1u : 0u) | (c < W - 1 ? 2u : 0u) | (c > 0 ? 4u : 0u) | (r < W - 1 ? 8u : 0u);
+}
+// 15-puzzle: the same masks, 4 bits per cell, from one 64-bit constant
+constexpr uint64_t legal4_table() {
+    uint64_t t = 0;
+    for (int c = 0; c < 16; ++c) {
+        const int r = c >> 2, k = c & 3;
+        const uint64_t m = (r > 0 ? 1u : 0u) | (k < 3 ? 2u : 0u) | (k > 0 ? 4u : 0u) | (r < 3 ? 8u : 0u);
+        t |= m << (4 * c);
+    }
+    return t;
+}
+template <int W, int HEUR, class BD> __device__ __forceinline__ uint32_t legal_moves(const Pos<W, HEUR, BD> &s) {
+    if constexpr (W == 4)
+        return (uint32_t)(legal4_table() >> (4 * s.bl)) & 15u;
+    else
+        return legal_moves<W>(s.br, s.bc);
+}
+
+// Manhattan move of the blank in direction d.  The tile slides one cell
+// along one axis, so h changes by exactly 1: down by 1 when the tile's goal
+// row (vertical move) or column (horizontal move) lies at or beyond the cell
+// it slides into.  Returns 1 when h went down, which is all that undoing the
+// move needs (unmove_md).
+template <int W, class BD>
+__device__ __forceinline__ uint32_t move_md(Pos<W, 0, BD> &s, int d) {
+    const int to = s.cell(), from = to + dir_cell<W>(d);
+    const int t = s.bd.tile(from);
+    const bool horiz = ((d ^ (d >> 1)) & 1) != 0; // right or left
+    int key, at;
+    if constexpr (W == 4) {
+        const int sh = horiz ? 0 : 2;
+        key = (t >> sh) & 3;
+        at = (to >> sh) & 3;
+    } else {
+        key = horiz ? tcol<W>(t) : trow<W>(t);
+        at = horiz ? s.col() : s.row();
+    }
+    // up / left: the tile moves to a larger row / column; right / down: smaller
+    const int ahead = (d & 1) ? at - key : key - at;
+    s.hs.h += ahead >= 0 ? -1 : 1;
+    s.bd.slide(t, from, to);
+    s.step(d);
+    return ahead >= 0 ? 1u : 0u;
+}
+
+// Undo of move_md(s, d) that returned `nearer`: the blank moves back.
+template <int W, class BD>
+__device__ __forceinline__ void unmove_md(Pos<W, 0, BD> &s, int d, uint32_t nearer) {
+    const int to = s.cell(), from = to - dir_cell<W>(d);
+    s.bd.slide(s.bd.tile(from), from, to);
+    s.hs.h += nearer ? 1 : -1;
+    s.step(3 - d);
+}
 
 // Move the blank one cell in direction d: the tile at the neighbour slides
 // into the blank.
 template <int W, int HEUR, class BD>
 __device__ __forceinline__ void do_move(Pos<W, HEUR, BD> &s, int d, const PdbCtx &P) {
-    int nr = s.br + (d == DIR_UP ? -1 : (d == DIR_DOWN ? 1 : 0));
-    int nc = s.bc + (d == DIR_LEFT ? -1 : (d == DIR_RIGHT ? 1 : 0));
-    int from = nr * W + nc, to = s.br * W + s.bc;
-    int t = s.bd.tile(from);
     if constexpr (HEUR == 0) {
-        int tr = trow<W>(t), tc = tcol<W>(t);
-        s.hs.h += abs(s.br - tr) + abs(s.bc - tc) - abs(nr - tr) - abs(nc - tc);
+        move_md(s, d);
     } else {
+        const int to = s.cell(), from = to + dir_cell<W>(d);
+        const int t = s.bd.tile(from);
         pdb_apply(P, s.hs, t, from, to);
+        s.bd.slide(t, from, to);
+        s.step(d);
     }
-    s.bd.slide(t, from, to);
-    s.br = nr;
-    s.bc = nc;
 }
 
 // Blank move that updates the board and blank only.  Queue items of the PDB
@@ -320,12 +396,9 @@ __device__ __forceinline__ void do_move(Pos<W, HEUR, BD> &s, int d, const PdbCtx
 // exported items are built with these moves and no table reads.
 template <int W, int HEUR, class BD>
 __device__ __forceinline__ void move_board_only(Pos<W, HEUR, BD> &s, int d) {
-    const int nr = s.br + dir_dr(d);
-    const int nc = s.bc + dir_dc(d);
-    const int from = nr * W + nc, to = s.br * W + s.bc;
+    const int to = s.cell(), from = to + dir_cell<W>(d);
     s.bd.slide(s.bd.tile(from), from, to);
-    s.br = nr;
-    s.bc = nc;
+    s.step(d);
 }
 
 // do_move for the PDB state with the moved tile's two lookup values known
@@ -437,12 +510,16 @@ __device__ __forceinline__ uint32_t pdb_moved_vals(const Pos<W, 1, BD> &s, int d
 // still to visit (always moves after d), and for the PDB search the values
 // the move overwrote, so that undoing it needs no table reads.
 template <int HEUR> struct StackEntry;
+// Manhattan: bits 0-1 the move, 2-5 the children still to visit, 6 whether
+// the move brought h down (move_md's result)
 template <> struct StackEntry<0> {
     using T = uint8_t;
-    static __device__ __forceinline__ T make(int d, uint32_t R, uint32_t) { return (T)((R << 2) | (uint32_t)d); }
+    static __device__ __forceinline__ T make(int d, uint32_t R, uint32_t v) {
+        return (T)((v << 6) | (R << 2) | (uint32_t)d);
+    }
     static __device__ __forceinline__ int dir(uint32_t e) { return (int)(e & 3u); }
-    static __device__ __forceinline__ uint32_t rest(uint32_t e) { return e >> 2; }
-    static __device__ __forceinline__ uint32_t vals(uint32_t) { return 0; }
+    static __device__ __forceinline__ uint32_t rest(uint32_t e) { return (e >> 2) & 15u; }
+    static __device__ __forceinline__ uint32_t vals(uint32_t e) { return e >> 6; }
 };
 template <> struct StackEntry<1> {
     using T = uint16_t;

@@ -90,7 +90,7 @@ __device__ __forceinline__ void put_item(const KParams &p, int r, unsigned long 
                                          int g, int prev, int inst, uint32_t origin) {
     Item it;
     s.bd.pack(it.b);
-    it.meta = meta_pack(s.br * W + s.bc, g, s.hs.h, prev);
+    it.meta = meta_pack(s.cell(), g, s.hs.h, prev);
     it.inst = (uint32_t)inst;
     it.x0 = 0;
     it.x1 = origin;
@@ -142,9 +142,7 @@ template <int W, int HEUR, class BD>
 __device__ __forceinline__ void pos_from_item(Pos<W, HEUR, BD> &s, const uint32_t (&b)[4], uint32_t meta,
                                               const PdbCtx &P) {
     s.bd.unpack(b);
-    int blank = meta_blank(meta);
-    s.br = blank / W;
-    s.bc = blank - s.br * W;
+    s.set_cell(meta_blank(meta));
     init_heur<W, HEUR>(s, P, meta);
 }
 
@@ -156,8 +154,12 @@ __device__ __forceinline__ Pos<W, HEUR> reg_pos(const Pos<W, HEUR, BD> &s) {
         c.bd = s.bd.regs();
     else
         c.bd = s.bd;
-    c.br = s.br;
-    c.bc = s.bc;
+    if constexpr (W == 4) {
+        c.bl = s.bl;
+    } else {
+        c.br = s.br;
+        c.bc = s.bc;
+    }
     c.hs = s.hs;
     return c;
 }
@@ -186,7 +188,9 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     const int NL = p.n_lanes;
     using SE = StackEntry<HEUR>;
     using ST = typename SE::T;
-    ST *stk = (ST *)lds + wave * 64 + lane; // level k at stk[k * sstride]
+    // level k at stk[k * sstride]; an LDS-qualified pointer keeps the
+    // address arithmetic in 32 bits
+    __attribute__((address_space(3))) ST *stk = (__attribute__((address_space(3))) ST *)lds + wave * 64 + lane;
     ST *cstk = (ST *)p.c_stack;
     const int sstride = nw * 64;
 
@@ -229,7 +233,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     } else {
         s.bd.unpack({0u, 0u, 0u, 0u});
     }
-    s.br = s.bc = 0;
+    s.set_cell(0);
     s.hs.h = 0;
     uint32_t root[4] = {0, 0, 0, 0};
     uint32_t rmeta = 0;
@@ -249,9 +253,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
         inst = (int)p.c_inst[gl];
         cnt = p.c_cnt[gl];
         origin = p.c_origin[gl];
-        int blank = meta_blank(m);
-        s.br = blank / W;
-        s.bc = blank - s.br * W;
+        s.set_cell(meta_blank(m));
         g = meta_g(m);
         prev = meta_prev(m);
         s.hs.h = meta_h(m);
@@ -302,7 +304,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
         if constexpr (HEUR == 1)
             ov = move_pdb<W>(s, d, P, kn, kvv);
         else
-            do_move<W, HEUR>(s, d, P);
+            ov = move_md(s, d);
         stk[sl * sstride] = SE::make(d, Rr, ov);
         sp = sl + 1;
         ++g;
@@ -392,7 +394,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
             if constexpr (HEUR == 1)
                 do_move_known<W>(s, 3 - d, P, SE::vals(e));
             else
-                do_move<W, HEUR>(s, 3 - d, P);
+                unmove_md(s, d, SE::vals(e));
             --g;
             if (Re) {
                 down = __builtin_ctz(Re);
@@ -464,10 +466,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
         int ng = 0;
         bool regfirst = true; // read the regular lookup first (else the reflected)
         if (st == ST_FRESH) {
-            const int br = s.br, bc = s.bc;
-            const uint32_t legal = (br > 0 ? 1u : 0u) | (bc < W - 1 ? 2u : 0u) | (bc > 0 ? 4u : 0u) |
-                                   (br < W - 1 ? 8u : 0u);
-            gen = legal & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
+            gen = legal_moves(s) & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
             if constexpr (HEUR == 1) {
                 // lane-slot j reads the lane's j-th generated child, so the
                 // wave runs as many slots as its largest family (3 below the
@@ -479,7 +478,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                 // src_horie/idas_pdb.cu:228-231, regular sum first).
                 ng = __builtin_popcount(gen);
                 regfirst = s.hs.A >= s.hs.B;
-                const int bl = br * W + bc;
+                const int bl = s.cell();
                 const PdbGeo &G = *P.geo;
                 uint32_t rem = gen;
 #pragma unroll
@@ -533,22 +532,31 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
             }
             if (xp) {
                 expanded = true;
-                const int br = s.br, bc = s.bc;
                 cnt += (uint32_t)__builtin_popcount(gen);
                 uint32_t pass = gen;
                 uint32_t cvals = 0, cvals2 = 0; // PDB values of children 0,1 / 2,3 (16 bits each)
-                const int bl = br * W + bc;
                 if constexpr (HEUR == 0) {
                     // Manhattan is consistent: h changes by exactly +-1 per
                     // move and f keeps the threshold's parity, so with slack
                     // >= 2 every child fits and with slack 0 only children
                     // with h-1 do.
                     if (L - g - h < 2) {
+                        const int br = s.row(), bc = s.col(), bl = s.cell();
+                        int tU, tD, tL, tR;
+                        if constexpr (W == 4) {
+                            // off-board neighbours wrap to some cell; gen
+                            // masks their bits
+                            tU = s.bd.tile((bl - 4) & 15);
+                            tD = s.bd.tile((bl + 4) & 15);
+                            tL = s.bd.tile((bl - 1) & 15);
+                            tR = s.bd.tile((bl + 1) & 15);
+                        } else {
+                            tU = s.bd.tile(br > 0 ? bl - W : bl);
+                            tD = s.bd.tile(br < W - 1 ? bl + W : bl);
+                            tL = s.bd.tile(bc > 0 ? bl - 1 : bl);
+                            tR = s.bd.tile(bc < W - 1 ? bl + 1 : bl);
+                        }
                         uint32_t imp = 0;
-                        int tU = s.bd.tile(br > 0 ? bl - W : bl);
-                        int tD = s.bd.tile(br < W - 1 ? bl + W : bl);
-                        int tL = s.bd.tile(bc > 0 ? bl - 1 : bl);
-                        int tR = s.bd.tile(bc < W - 1 ? bl + 1 : bl);
                         imp |= (trow<W>(tU) >= br) ? 1u : 0u;
                         imp |= (tcol<W>(tR) <= bc) ? 2u : 0u;
                         imp |= (tcol<W>(tL) >= bc) ? 4u : 0u;
@@ -710,9 +718,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
             Pos<W, HEUR> c;
             if constexpr (HEUR == 1) {
                 c.bd.unpack(root);
-                const int blank = meta_blank(rmeta);
-                c.br = blank / W;
-                c.bc = blank - c.br * W;
+                c.set_cell(meta_blank(rmeta));
                 c.hs.h = 0;
             } else {
                 pos_from_item<W, HEUR>(c, root, rmeta, P);
@@ -789,7 +795,7 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
         for (int k = 0; k < 4; ++k) p.c_rows[k * NL + gl] = b[k];
 #pragma unroll
         for (int k = 0; k < 4; ++k) p.c_root[k * NL + gl] = root[k];
-        p.c_meta[gl] = meta_pack(s.br * W + s.bc, g, s.hs.h, prev);
+        p.c_meta[gl] = meta_pack(s.cell(), g, s.hs.h, prev);
         p.c_rmeta[gl] = rmeta;
         p.c_inst[gl] = (uint32_t)inst;
         p.c_cnt[gl] = cnt;
@@ -880,8 +886,7 @@ __device__ unsigned split_node(const PdbCtx &P, const Item &x, int L, int levels
     pos_from_item<W, HEUR>(s, x.b, x.meta, P);
     const int g = meta_g(x.meta), prev = meta_prev(x.meta);
     unsigned gen_total = 0;
-    uint32_t gen = (s.br > 0 ? 1u : 0u) | (s.bc < W - 1 ? 2u : 0u) | (s.bc > 0 ? 4u : 0u) |
-                   (s.br < W - 1 ? 8u : 0u);
+    uint32_t gen = legal_moves(s);
     gen &= ~(prev < 4 ? (1u << (3 - prev)) : 0u);
     gen_total += (unsigned)__builtin_popcount(gen);
 #pragma unroll 1
@@ -895,8 +900,7 @@ __device__ unsigned split_node(const PdbCtx &P, const Item &x, int L, int levels
             emit(c, g + 1, d, 1, d, 0);
             continue;
         }
-        uint32_t gen2 = (c.br > 0 ? 1u : 0u) | (c.bc < W - 1 ? 2u : 0u) | (c.bc > 0 ? 4u : 0u) |
-                        (c.br < W - 1 ? 8u : 0u);
+        uint32_t gen2 = legal_moves(c);
         gen2 &= ~(1u << (3 - d));
         gen_total += (unsigned)__builtin_popcount(gen2);
 #pragma unroll 1
@@ -1092,7 +1096,7 @@ __global__ void __launch_bounds__(256) k_restart(KParams p, const PdbGeo *geo_g)
                         unsigned k = atomicAdd(&s_bucket[split_bucket(b, level)], 1u);
                         Item ci;
                         cp.bd.pack(ci.b);
-                        ci.meta = meta_pack(cp.br * W + cp.bc, g, cp.hs.h, prev);
+                        ci.meta = meta_pack(cp.cell(), g, cp.hs.h, prev);
                         ci.inst = (uint32_t)isl;
                         ci.x0 = 0;
                         ci.x1 = i;
@@ -1248,7 +1252,6 @@ __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *ge
     Pos<W, HEUR> s;
     pos_from_item<W, HEUR>(s, it.b, it.meta, P);
     int g = meta_g(it.meta), prev = meta_prev(it.meta);
-    const int br = s.br, bc = s.bc;
     // F = largest f on the path from the root (x0 carries the parent's F):
     // the node is expanded in iteration L iff F <= L, heuristic consistent
     // or not
@@ -1264,8 +1267,7 @@ __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *ge
                 if (atomicCAS(&p.sol_len[sl], -1, g) == -1)
                     path_from_u64(lp_in[(size_t)inst * p.capl + j], p.sol, (uint32_t)sl * kPathWords);
     }
-    uint32_t legal = (br > 0 ? 1u : 0u) | (bc < W - 1 ? 2u : 0u) | (bc > 0 ? 4u : 0u) | (br < W - 1 ? 8u : 0u);
-    uint32_t gen = legal & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
+    uint32_t gen = legal_moves(s) & ~(prev < 4 ? (1u << (3 - prev)) : 0u);
     atomicAdd(&topgen[(size_t)inst * kFBins + (f < kFBins ? f : kFBins - 1)], (unsigned)__builtin_popcount(gen));
     unsigned k = atomicAdd(&n_out[inst], (unsigned)__builtin_popcount(gen));
     while (gen) {
@@ -1275,7 +1277,7 @@ __global__ void __launch_bounds__(256) k_split_level(KParams p, const PdbGeo *ge
         do_move<W, HEUR>(c, d, P);
         Item ci;
         c.bd.pack(ci.b);
-        ci.meta = meta_pack(c.br * W + c.bc, g + 1, c.hs.h, d);
+        ci.meta = meta_pack(c.cell(), g + 1, c.hs.h, d);
         ci.inst = (uint32_t)inst;
         ci.x0 = (uint32_t)f;
         ci.x1 = 0;
