@@ -332,11 +332,44 @@ constexpr uint64_t legal4_table() {
     }
     return t;
 }
+template <int W> __device__ __forceinline__ uint32_t legal_cell(int c) {
+    if constexpr (W == 4)
+        return (uint32_t)(legal4_table() >> (4 * c)) & 15u;
+    else
+        return legal_moves<W>(trow<W>(c), tcol<W>(c));
+}
 template <int W, int HEUR, class BD> __device__ __forceinline__ uint32_t legal_moves(const Pos<W, HEUR, BD> &s) {
     if constexpr (W == 4)
-        return (uint32_t)(legal4_table() >> (4 * s.bl)) & 15u;
+        return legal_cell<4>(s.bl);
     else
         return legal_moves<W>(s.br, s.bc);
+}
+
+// Manhattan: the moves of a blank at cell c that slide a tile nearer its
+// goal (h - 1), bit d for direction d.  Bits of moves off the board are
+// meaningless: callers mask them with the legal moves.
+template <int W, class BD> __device__ __forceinline__ uint32_t nearer_moves(const BD &bd, int c) {
+    const int r = trow<W>(c), k = tcol<W>(c);
+    int tU, tD, tL, tR;
+    if constexpr (W == 4) {
+        // the board rotated so that cell c - 4 is at bits 0-3: the four
+        // neighbours sit at bits 0, 12, 20 and 32 (off-board ones wrap to
+        // some cell)
+        const uint32_t sh = (uint32_t)(4 * c - 16) & 63u;
+        const uint64_t w = (bd.b >> sh) | (bd.b << ((64u - sh) & 63u));
+        const uint32_t lo = (uint32_t)w;
+        tU = (int)(lo & 15u);
+        tL = (int)((lo >> 12) & 15u);
+        tR = (int)((lo >> 20) & 15u);
+        tD = (int)((uint32_t)(w >> 32) & 15u);
+    } else {
+        tU = bd.tile(r > 0 ? c - W : c);
+        tD = bd.tile(r < W - 1 ? c + W : c);
+        tL = bd.tile(k > 0 ? c - 1 : c);
+        tR = bd.tile(k < W - 1 ? c + 1 : c);
+    }
+    return (trow<W>(tU) >= r ? 1u : 0u) | (tcol<W>(tR) <= k ? 2u : 0u) | (tcol<W>(tL) >= k ? 4u : 0u) |
+           (trow<W>(tD) <= r ? 8u : 0u);
 }
 
 // Manhattan move of the blank in direction d.  The tile slides one cell

@@ -540,28 +540,30 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
                     // move and f keeps the threshold's parity, so with slack
                     // >= 2 every child fits and with slack 0 only children
                     // with h-1 do.
-                    if (L - g - h < 2) {
-                        const int br = s.row(), bc = s.col(), bl = s.cell();
-                        int tU, tD, tL, tR;
-                        if constexpr (W == 4) {
-                            // off-board neighbours wrap to some cell; gen
-                            // masks their bits
-                            tU = s.bd.tile((bl - 4) & 15);
-                            tD = s.bd.tile((bl + 4) & 15);
-                            tL = s.bd.tile((bl - 1) & 15);
-                            tR = s.bd.tile((bl + 1) & 15);
-                        } else {
-                            tU = s.bd.tile(br > 0 ? bl - W : bl);
-                            tD = s.bd.tile(br < W - 1 ? bl + W : bl);
-                            tL = s.bd.tile(bc > 0 ? bl - 1 : bl);
-                            tR = s.bd.tile(bc < W - 1 ? bl + 1 : bl);
+                    const int slack = L - g - h;
+                    if (slack <= 2) {
+                        const uint32_t near = nearer_moves<W>(s.bd, s.cell());
+                        if (slack < 2) pass = gen & near;
+                        // A child at slack 0 (slack 0 and nearer, or slack 2
+                        // and farther) whose own children all fail the
+                        // f-test is a leaf: its children are counted here
+                        // and it is not entered.  Its children's tiles are
+                        // all on the current board (entering it would only
+                        // move the blank there).  With h >= 2 no such child
+                        // is a goal, and a leaf has no goal child (one would
+                        // be nearer, and pass).  One child per expansion is
+                        // tested, the first such: a wave would run the test
+                        // as often as its lane with the most.
+                        const uint32_t cand = h < 2 ? 0u : (slack < 2 ? pass : gen & ~near);
+                        if (cand) {
+                            const int d = __builtin_ctz(cand);
+                            const int cc = s.cell() + dir_cell<W>(d);
+                            const uint32_t cgen = legal_cell<W>(cc) & ~(1u << (3 - d));
+                            if (!(cgen & nearer_moves<W>(s.bd, cc))) {
+                                cnt += (uint32_t)__builtin_popcount(cgen);
+                                pass &= ~(1u << d);
+                            }
                         }
-                        uint32_t imp = 0;
-                        imp |= (trow<W>(tU) >= br) ? 1u : 0u;
-                        imp |= (tcol<W>(tR) <= bc) ? 2u : 0u;
-                        imp |= (tcol<W>(tL) >= bc) ? 4u : 0u;
-                        imp |= (trow<W>(tD) <= br) ? 8u : 0u;
-                        pass = gen & imp;
                     }
                 } else {
                     // the PDB max is not consistent (lookups take the minimum
