@@ -71,11 +71,14 @@ def test_lookahead_24_puzzle_pdb(ref_sets, ref_results):
 
 def test_lookahead_first_solution_mode(golden):
     """First-solution mode: the last iteration stops at a goal, so its count
-    depends on the schedule; lengths and every earlier count do not."""
+    depends on the schedule; lengths and every earlier count do not.  Few
+    lanes: lookahead starts only after an iteration of one node per lane."""
     fx = golden("iterations_c1_walk40.json")["instances"][:40]
     boards = [r["board"] for r in fx] + [O.walk(4, 70, 5_300_000 + i) for i in range(20)]
-    on, _ = results(boards, width=4, mode=spz.MODE_FIRST)
-    off, _ = results(boards, width=4, mode=spz.MODE_FIRST, lookahead=False)
+    on, st = results(boards, width=4, mode=spz.MODE_FIRST, lanes=8192)
+    off, _ = results(boards, width=4, mode=spz.MODE_FIRST, lanes=8192, lookahead=False)
+    say("lookahead starts", st["lookahead_starts"])
+    assert st["lookahead_starts"] > 0
     for (o1, n1, c1), (o2, n2, c2) in zip(on, off):
         assert (o1, n1) == (o2, n2)
         assert c1[:n1 - 1] == c2[:n2 - 1]
@@ -83,13 +86,33 @@ def test_lookahead_first_solution_mode(golden):
 
 def test_lookahead_iteration_limit():
     """An iteration cap ends instances unsolved the same way with and
-    without lookahead (no count of an iteration past the cap)."""
-    boards = [O.walk(4, 60, 5_400_000 + i) for i in range(24)]
-    on, _ = results(boards, max_iters=3, width=4)
-    off, _ = results(boards, max_iters=3, width=4, lookahead=False)
+    without lookahead (no count of an iteration past the cap).  Random
+    boards (iterations of 10^3..10^7 nodes) on few lanes, so that lookahead
+    starts up to the cap."""
+    boards = [O.random_solvable(4, 5_400_000 + i) for i in range(24)]
+    on, st = results(boards, max_iters=6, width=4, lanes=4096)
+    off, _ = results(boards, max_iters=6, width=4, lanes=4096, lookahead=False)
+    say("lookahead starts", st["lookahead_starts"])
     assert on == off
+    assert st["lookahead_starts"] > 0
     assert any(o == -1 for o, _, _ in on)
-    assert all(n <= 3 for _, n, _ in on)
+    assert all(n <= 6 for _, n, _ in on)
+
+
+def test_lookahead_skips_small_iterations(golden):
+    """Instances whose last iteration counted fewer nodes than there are
+    lanes start no lookahead past their first iteration: 40-move walks
+    (a few hundred thousand nodes each) on the default lane count get at
+    most one lookahead start each, and their counts stay the reference's."""
+    fx = golden("iterations_c1_walk40.json")["instances"]
+    on, st = results([r["board"] for r in fx], width=4)
+    say("lookahead starts", st["lookahead_starts"], "instances", len(fx))
+    assert st["lookahead_starts"] <= len(fx)
+    for rec, (opt, _, counts) in zip(fx, on):
+        assert opt == rec["opt"]
+        for a, b in zip(counts, rec["counts"]):
+            if b is not None:
+                assert a == b, rec["idx"]
 
 
 def test_lookahead_off_under_load_policy():
