@@ -90,6 +90,11 @@ WORKLOAD = {
 }
 
 
+# a span of the direct tables that every XCD's L2 (4 MB) holds: gathers over
+# it measure the L2-hit side of the random-read ceiling
+L2_RESIDENT_SPAN = 2 << 20
+
+
 def profile_figure(config: str, kind: str, what: str, key: str):
     """A per-node figure of k_round measured by rocprofv3 --pmc passes on this
     workload and kept in its profiles/ summary k_round_<config>_<kind>_<what>.md
@@ -504,11 +509,25 @@ def main():
     if pdb_ms is not None:
         out["pdb_build_ms"] = pdb_ms
         _, gps = solver.gather_probe(1 << 30)
+        _, gps_l2 = solver.gather_probe(1 << 30, span_bytes=L2_RESIDENT_SPAN)
         roof["measured_gather_peak"] = {
             "gathers_per_s": gps, "GB_per_s_at_64B": gps * 64 / 1e9,
-            "what": "uniform random u8 gathers over the two 244 MB direct tables (spz_gather_probe)"}
+            "what": "uniform random u8 gathers over the two 244 MB direct tables (spz_gather_probe)",
+            "l2_resident_gathers_per_s": gps_l2,
+            "l2_resident_what": f"the same over the tables' first {L2_RESIDENT_SPAN >> 20} MB (L2-resident)"}
         if "l2_miss_rate_per_s" in roof:
             roof["measured_gather_peak"]["l2_miss_rate_frac_of_it"] = roof["l2_miss_rate_per_s"] / gps
+        hits = profile_figure(pcfg, "pdb", "traffic", "l2_hits_per_node")
+        if hits is not None and "l2_miss_rate_per_s" in roof:
+            # the table reads' share of the memory system's random-read
+            # capacity: each L2 hit takes 1/gps_l2 s of it, each miss 1/gps
+            hit_rate = hits * nodes_per_launch / avg_launch_s
+            roof["gather_capacity"] = {
+                "l2_hits_per_node": hits, "l2_misses_per_node": roof["l2_misses_per_node"],
+                "reads_per_s": hit_rate + roof["l2_miss_rate_per_s"],
+                "frac": hit_rate / gps_l2 + roof["l2_miss_rate_per_s"] / gps,
+                "what": "L2 hits / L2-resident gather rate + L2 misses / whole-table gather rate, "
+                        "per second of launch time (1.0 = the measured random-read ceiling at the kernel's hit mix)"}
 
     if not args.no_plain and not per_instance:
         # plain GPU IDA*: one instance per launch, uniform static split, one

@@ -279,10 +279,13 @@ int spz_last_stats(spz_ctx *ctx, double *stats);
 const char *spz_last_error(spz_ctx *ctx);
 void spz_destroy(spz_ctx *ctx);
 
-/* Diagnostics: uniformly random byte gathers over the two direct tables
- * (the search's access pattern without the search) -- the measured
- * random-access ceiling for the PDB kernel.  No reference counterpart. */
-int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, double *ms, double *gathers_per_s);
+/* Diagnostics: uniformly random byte gathers over the first span_bytes of
+ * the two direct tables (span_bytes <= 0: all 2 x 25^6 bytes) -- the
+ * search's access pattern without the search.  Over the whole tables nearly
+ * every gather misses L2; over a span of a few MB nearly every one hits:
+ * the two measured random-access ceilings of the PDB kernel.  No reference
+ * counterpart. */
+int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, int64_t span_bytes, double *ms, double *gathers_per_s);
 
 /* Diagnostics: the eight PDB lookups as (tiles in slot order, cell map),
  * derived from the cell symmetries; the reference writes them out as

@@ -36,8 +36,8 @@ hipError_t launch_eval_h_bytes(int width, int heur, const uint8_t *bytes, int n,
 hipError_t pdb_bfs_build(int pat, uint8_t *compact, uint32_t *work, unsigned long long *d_cnt, hipStream_t s,
                          int *levels_out);
 hipError_t pdb_spread(const uint8_t *compact, uint8_t *direct, hipStream_t s);
-hipError_t launch_gather_probe(const uint8_t *t0, const uint8_t *t1, int blocks, int per_thread, uint64_t seed,
-                               uint32_t *out, hipStream_t s);
+hipError_t launch_gather_probe(const uint8_t *t, uint64_t span, int blocks, int per_thread, uint64_t seed, uint32_t *out,
+                               hipStream_t s);
 constexpr uint64_t kPlacementsHost = 127512000ull;
 uint32_t pdb_goal_rank(int pattern);
 } // namespace spz
@@ -1356,9 +1356,11 @@ int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens) 
     return SPZ_OK;
 }
 
-int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, double *ms, double *gathers_per_s) {
+int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, int64_t span_bytes, double *ms, double *gathers_per_s) {
     if (!ctx || !ms || !gathers_per_s || n_gathers <= 0) return SPZ_E_ARG;
     if (!ctx->have_pdb) return fail(ctx, SPZ_E_NOPDB, "call spz_build_pdb first");
+    const uint64_t both = 2ull * kPdbDirect; // the two direct tables, end to end
+    const uint64_t span = (span_bytes <= 0 || (uint64_t)span_bytes > both) ? both : (uint64_t)span_bytes;
     HIPCHK(hipSetDevice(ctx->cfg.device));
     const int per_thread = 256;
     long long threads = (n_gathers + per_thread - 1) / per_thread;
@@ -1368,12 +1370,12 @@ int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, double *ms, double *gather
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
-    hipError_t e = launch_gather_probe(ctx->pdb_direct.as<uint8_t>(), ctx->pdb_direct.as<uint8_t>() + kPdbDirect, blocks,
-                                       per_thread, 12345, out.as<uint32_t>(), ctx->stream); // warm
+    hipError_t e = launch_gather_probe(ctx->pdb_direct.as<uint8_t>(), span, blocks, per_thread, 12345,
+                                       out.as<uint32_t>(), ctx->stream); // warm
     if (e == hipSuccess) e = hipEventRecord(a, ctx->stream);
     if (e == hipSuccess)
-        e = launch_gather_probe(ctx->pdb_direct.as<uint8_t>(), ctx->pdb_direct.as<uint8_t>() + kPdbDirect, blocks,
-                                per_thread, 777, out.as<uint32_t>(), ctx->stream);
+        e = launch_gather_probe(ctx->pdb_direct.as<uint8_t>(), span, blocks, per_thread, 777, out.as<uint32_t>(),
+                                ctx->stream);
     if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
     if (e == hipSuccess) e = hipEventSynchronize(b);
     float t = 0;

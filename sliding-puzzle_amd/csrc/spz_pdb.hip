@@ -233,8 +233,8 @@ __global__ void __launch_bounds__(256) k_pdb_spread(const uint8_t *__restrict__ 
 // Random-gather probe: every thread reads `per_thread` bytes at uniformly
 // random positions of the two direct tables (independent addresses, so many
 // are in flight), the access pattern of the PDB search without its search.
-__global__ void __launch_bounds__(256) k_gather_probe(const uint8_t *__restrict__ t0, const uint8_t *__restrict__ t1,
-                                                      int per_thread, uint64_t seed, uint32_t *out) {
+__global__ void __launch_bounds__(256) k_gather_probe(const uint8_t *__restrict__ t, uint64_t span, int per_thread,
+                                                      uint64_t seed, uint32_t *out) {
     uint64_t x = seed ^ ((uint64_t)(blockIdx.x * blockDim.x + threadIdx.x) * 0x9e3779b97f4a7c15ull);
     uint32_t acc = 0;
     for (int k = 0; k < per_thread; k += 4) {
@@ -245,16 +245,17 @@ __global__ void __launch_bounds__(256) k_gather_probe(const uint8_t *__restrict_
             x ^= x << 25;
             x ^= x >> 27;
             uint64_t r = x * 0x2545f4914f6cdd1dull;
-            i[j] = (uint32_t)(((r >> 32) * kPdbDirect) >> 32);
+            i[j] = (uint32_t)(((r >> 32) * span) >> 32); // < span
         }
-        acc += t0[i[0]] + t1[i[1]] + t0[i[2]] + t1[i[3]];
+        acc += t[i[0]] + t[i[1]] + t[i[2]] + t[i[3]];
     }
     out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
-hipError_t launch_gather_probe(const uint8_t *t0, const uint8_t *t1, int blocks, int per_thread, uint64_t seed,
-                               uint32_t *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_gather_probe, dim3(blocks), dim3(256), 0, s, t0, t1, per_thread, seed, out);
+// span < 2^32 bytes (the two direct tables are 2 x 25^6)
+hipError_t launch_gather_probe(const uint8_t *t, uint64_t span, int blocks, int per_thread, uint64_t seed, uint32_t *out,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_probe, dim3(blocks), dim3(256), 0, s, t, span, per_thread, seed, out);
     return hipGetLastError();
 }
 

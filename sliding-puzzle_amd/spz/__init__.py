@@ -110,7 +110,7 @@ def lib() -> ctypes.CDLL:
     L.spz_last_error.restype = ctypes.c_char_p
     L.spz_destroy.argtypes = [vp]
     L.spz_destroy.restype = None
-    L.spz_gather_probe.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
+    L.spz_gather_probe.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_double)]
     f32p, i32p = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int32)
     L.spz_mlp_param_count.argtypes = []
@@ -265,9 +265,11 @@ class Solver:
         self._check(lib().spz_get_pdb(self._ctx, pattern, _u8p(out)), "spz_get_pdb")
         return out
 
-    def gather_probe(self, n_gathers: int = 1 << 30):
+    def gather_probe(self, n_gathers: int = 1 << 30, span_bytes: int = 0):
+        """Random byte gathers over the first span_bytes of the direct tables
+        (0: all of them).  Returns (ms, gathers per second)."""
         ms, gps = ctypes.c_double(), ctypes.c_double()
-        self._check(lib().spz_gather_probe(self._ctx, n_gathers, ctypes.byref(ms), ctypes.byref(gps)),
+        self._check(lib().spz_gather_probe(self._ctx, n_gathers, span_bytes, ctypes.byref(ms), ctypes.byref(gps)),
                     "spz_gather_probe")
         return ms.value, gps.value
 
