@@ -71,7 +71,9 @@ typedef struct spz_config {
     int32_t max_split_depth; /* cap on that split depth (0 = 24) */
     int32_t workgroup_size;  /* 64, 128 or 256 lanes (0 = 256) */
     int32_t lanes;           /* DFS lanes in flight (0 = fill the device) */
-    int32_t round_budget;    /* DFS steps per lane per round (0 = auto) */
+    int32_t round_budget;    /* DFS steps per lane per round (0 = auto:
+                                2048 for Manhattan when the batch has work
+                                for every lane, else 1024) */
     int32_t lds_stack_depth; /* DFS stack levels per lane kept in LDS (0 =
                                 80 for width 4, 96 for width 5 Manhattan, 64
                                 for the PDB search; at most 160);
@@ -104,8 +106,9 @@ typedef struct spz_config {
                                 microseconds after its start (whichever comes
                                 first with round_budget steps), so that waves
                                 that step faster under load do not wait for
-                                the slowest; 0 = auto (4500 for the PDB search,
-                                2500 for Manhattan), < 0 = off */
+                                the slowest; 0 = auto (4500 for the PDB search;
+                                Manhattan 3500 when the batch has work for
+                                every lane, else 2500), < 0 = off */
     int32_t dedup;           /* 1: duplicate pruning in the root split: a
                                 top-tree node whose board an earlier kept node
                                 of its instance holds with a path-max f no

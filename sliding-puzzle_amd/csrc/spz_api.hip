@@ -238,9 +238,13 @@ void spz_default_config(spz_config *cfg, int32_t width, int32_t heuristic) {
 
 // Steps per lane and round when the configuration leaves it open: shorter
 // rounds refill idle lanes sooner, longer ones spend less on the context save
-// and restore (measured on MI355X, both heuristics: 1024 beats 256 and 512,
-// 2048 is level with it).
-static int default_round_budget(int) { return 1024; }
+// and restore.  `full`: the solve has work for every lane of the context (its
+// automatic lane count reached the capacity).  Measured on MI355X: 1024 beats
+// 256 and 512 for the PDB search, where 2048 is level with it; the Manhattan
+// search gains from longer rounds only when the batch fills every lane (4096
+// mixed 15-puzzles: 2048 steps / 3.5 ms 829 instances/s, 1024 / 2.5 ms 773),
+// and loses with one instance on a part of them (44.8 -> 39.6).
+static int default_round_budget(int heur, bool full) { return (heur != SPZ_H_PDB666 && full) ? 2048 : 1024; }
 
 // Time limit of a round when the configuration leaves it open.  With a step
 // budget alone, waves that step faster (fewer lanes on divergent paths) end
@@ -248,8 +252,11 @@ static int default_round_budget(int) { return 1024; }
 // waves end 4.75 ms (10th percentile) to 5.5 ms (last) into a 1024-step
 // round.  The limit cuts that tail; lightly loaded rounds, which step faster,
 // still end at their step budget (measured on MI355X: PDB 4500 us beat 3200,
-// 4000, 4800 and 5600; Manhattan 2500 us beat 1200, 2000 and 3000).
-static int default_round_time_us(int heur) { return heur == SPZ_H_PDB666 ? 4500 : 2500; }
+// 4000, 4800 and 5600; Manhattan 2500 us beat 1200, 2000 and 3000 at 1024
+// steps, and 3500 us 2000 and 3000 at 2048).
+static int default_round_time_us(int heur, bool full) {
+    return heur == SPZ_H_PDB666 ? 4500 : (full ? 3500 : 2500);
+}
 
 // Lanes per compute unit of the 24-puzzle searches: more resident lanes than
 // this lose more to L2 sharing and finer work splitting than they gain in
@@ -446,7 +453,9 @@ int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_ite
         return SPZ_OK;
     };
     spz_config bc = base;
-    if (bc.round_budget <= 0) bc.round_budget = default_round_budget(bc.heuristic);
+    if (bc.round_budget <= 0)
+        bc.round_budget = default_round_budget(bc.heuristic, kLanesPerItem * (size_t)n * (size_t)std::max(1, bc.frontier_target) >=
+                                                                 (size_t)ctx->n_lanes);
     double bms = 0;
     int count = 0;
     int rc = run(bc, &bms, false); // reference result (also warms the buffers)
@@ -1040,10 +1049,11 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         memset(ctl, 0, sizeof ctl);
         ctl[0] = (unsigned)n; // restart list length
         ctl[1] = (unsigned)n; // active instances
-        int big = ctx->cfg.round_budget > 0 ? ctx->cfg.round_budget : default_round_budget(heur);
+        const bool full = lanes >= (size_t)ctx->n_lanes;
+        int big = ctx->cfg.round_budget > 0 ? ctx->cfg.round_budget : default_round_budget(heur, full);
         // a round ends after its step budget or its time limit, whichever
         // comes first
-        const int rt = ctx->cfg.round_time_us == 0 ? default_round_time_us(heur) : ctx->cfg.round_time_us;
+        const int rt = ctx->cfg.round_time_us == 0 ? default_round_time_us(heur, full) : ctx->cfg.round_time_us;
         p.round_ticks = (rt > 0 && ctx->wall_ticks_per_us > 0)
                             ? (unsigned long long)(rt * ctx->wall_ticks_per_us) : 0ull;
         ctl[9] = (unsigned)big;
