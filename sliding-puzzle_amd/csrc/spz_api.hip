@@ -1,6 +1,7 @@
 // spz_api.hip -- host implementation of the libspz C ABI (include/spz.h).
 #include "../../include/spz.h"
 #include "spz_board.h"
+#include "spz_dedup.h"
 #include "spz_mlp.h"
 
 #include <stdarg.h>
@@ -726,22 +727,7 @@ int spz_export_pdb_tab(spz_ctx *ctx, const char *p0, const char *p1) {
 // would be, for any admissible heuristic, consistent or not.  Optimal
 // lengths stay; the dropped node's subtree is no longer counted.  Host side,
 // between split levels: it changes the set-up only, never the search
-// kernels.
-struct BoardKey {
-    uint32_t w[4];
-    bool operator<(const BoardKey &o) const {
-        for (int i = 0; i < 4; ++i)
-            if (w[i] != o.w[i]) return w[i] < o.w[i];
-        return false;
-    }
-    bool operator==(const BoardKey &o) const {
-        return w[0] == o.w[0] && w[1] == o.w[1] && w[2] == o.w[2] && w[3] == o.w[3];
-    }
-};
-// an instance's kept top-tree boards with the smallest path-max f kept for
-// each, sorted by board
-using SeenBoards = std::vector<std::pair<BoardKey, int>>;
-
+// kernels.  The per-instance pass is dedup_level_host (spz_dedup.h).
 static BoardKey board_key(int width, const Item &it) {
     return BoardKey{{it.b[0], it.b[1], width == 4 ? 0u : it.b[2], width == 4 ? 0u : it.b[3]}};
 }
@@ -1271,71 +1257,11 @@ static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur,
         e = hipMemcpy2DAsync(lp, W * 8, ctx->lpath[cur].p, C * 8, W * 8, N, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
-    // within a level the copy kept is the one with the smallest (path-max f,
-    // previous move): the level's order comes from the split kernel's atomics,
-    // and copies equal in both are the same node, so the result does not
-    // depend on that order.  Per instance: sort the level by (board, f,
-    // previous move), take each board's first entry, merge-join it with the
-    // boards kept at shallower levels.  Instances are independent and split
-    // over host threads.
-    struct Entry {
-        BoardKey k;
-        int f, prev;
-        uint32_t j;
-        bool operator<(const Entry &o) const {
-            if (!(k == o.k)) return k < o.k;
-            if (f != o.f) return f < o.f;
-            return prev < o.prev;
-        }
-    };
-    auto one = [&](size_t i) {
-        const size_t cnt = std::min<size_t>(nl[i], C);
-        std::vector<Entry> v(cnt);
-        for (size_t j = 0; j < cnt; ++j) {
-            const Item &it = items[i * W + j];
-            v[j].k = board_key(width, it);
-            v[j].f = std::max((int)it.x0, meta_g(it.meta) + meta_h(it.meta));
-            v[j].prev = meta_prev(it.meta);
-            v[j].j = (uint32_t)j;
-        }
-        std::sort(v.begin(), v.end());
-        std::vector<char> keep(cnt, 0);
-        SeenBoards &sn = seen[i];
-        SeenBoards merged;
-        merged.reserve(sn.size() + cnt);
-        size_t a = 0;
-        for (size_t x = 0; x < cnt; ++x) {
-            if (x > 0 && v[x].k == v[x - 1].k) continue; // not the board's best copy
-            while (a < sn.size() && sn[a].first < v[x].k) merged.push_back(sn[a++]);
-            if (a < sn.size() && sn[a].first == v[x].k) {
-                if (sn[a].second <= v[x].f) continue; // a shallower kept copy covers it
-                merged.emplace_back(v[x].k, v[x].f);
-                ++a;
-            } else {
-                merged.emplace_back(v[x].k, v[x].f);
-            }
-            keep[v[x].j] = 1;
-        }
-        while (a < sn.size()) merged.push_back(sn[a++]);
-        sn.swap(merged);
-        size_t out = 0;
-        for (size_t j = 0; j < cnt; ++j) {
-            if (!keep[j]) continue;
-            if (out != j) {
-                items[i * W + out] = items[i * W + j];
-                if (paths) lp[i * W + out] = lp[i * W + j];
-            }
-            ++out;
-        }
-        nl[i] = (unsigned)out;
-    };
-    const size_t nth = std::min<size_t>(N, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
-    std::vector<std::thread> pool;
-    for (size_t t = 0; t < nth; ++t)
-        pool.emplace_back([&, t]() {
-            for (size_t i = t; i < N; i += nth) one(i);
-        });
-    for (auto &th : pool) th.join();
+    dedup_level_host(
+        items, paths ? lp : nullptr, N, W, nl.data(), seen, [&](const Item &it) { return board_key(width, it); },
+        [](const Item &it) { return std::max((int)it.x0, meta_g(it.meta) + meta_h(it.meta)); },
+        [](const Item &it) { return meta_prev(it.meta); },
+        std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
     e = hipMemcpyAsync(ctx->nlvl[cur].p, nl.data(), N * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
         e = hipMemcpy2DAsync(ctx->lvl[cur].p, C * sizeof(Item), items, W * sizeof(Item), W * sizeof(Item), N,

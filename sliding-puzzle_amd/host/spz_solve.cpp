@@ -8,6 +8,7 @@
 // src_horie/idas_cpu_expand.c:288-304 ("f_limit=%3d, expanded nodes = %lld"),
 // --csv writes per-instance seconds like the reference's result/*.csv files.
 #include "spz.h"
+#include "spz_problem.h"
 
 #include <chrono>
 #include <cstdio>
@@ -41,21 +42,8 @@ static void usage() {
                  "  --dedup                drop duplicate boards from the root split (fewer nodes;\n"
                  "                         not the reference's counts)\n"
                  "  --round-time US        a search round also ends after US microseconds (-1: off;\n"
-                 "                         default 4500 for the PDB search, 2500 for Manhattan)\n");
-}
-
-static bool read_board(const std::string &path, std::vector<uint8_t> &b, int &width) {
-    std::ifstream f(path);
-    if (!f) return false;
-    std::vector<int> v;
-    int x;
-    while (f >> x) v.push_back(x);
-    int w = v.size() >= 25 ? 5 : (v.size() >= 16 ? 4 : 0);
-    if (w == 0) return false;
-    if (width == 0) width = w;
-    if (w != width) return false;
-    for (int i = 0; i < width * width; ++i) b.push_back((uint8_t)v[i]);
-    return true;
+                 "                         default 4500 for the PDB search; Manhattan 3500 when the\n"
+                 "                         batch has work for every lane, else 2500)\n");
 }
 
 int main(int argc, char **argv) {
@@ -145,8 +133,9 @@ int main(int argc, char **argv) {
     int width = heur == SPZ_H_PDB666 ? 5 : 0;
     std::vector<uint8_t> boards;
     for (auto &f : files)
-        if (!read_board(f, boards, width)) {
-            std::fprintf(stderr, "spz_solve: cannot read a %s board from %s\n", width ? "matching" : "", f.c_str());
+        if (!read_problem_file(f, boards, width)) {
+            std::fprintf(stderr, "spz_solve: %s: not a %s board (N*N integers 0..N*N-1, 0 = blank)\n", f.c_str(),
+                         width == 5 ? "5x5" : width == 4 ? "4x4" : "4x4 or 5x5");
             return 1;
         }
     if (width == 0) width = 5;
