@@ -215,12 +215,12 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
     // stack levels, then (width 5) the lanes' boards
     Pos<W, HEUR, typename HotBoard<W>::T> s;
     // Sibling-value cache (PDB): when a lane enters the first passing child
-    // of a node, the values its f-test read for the next two passing
-    // children go into slot level % kSibLevels (tag = level in bits 24-31,
-    // 6+6 bits per sibling); moving into such a sibling later then reads no
-    // table.  An entry overwritten by a deeper level (or from an earlier
-    // round: LDS does not persist) fails its tag and the move reads the
-    // tables as before.
+    // of a node with at most two more, the values its f-test read for them
+    // go into slot level % kSibLevels (tag = level in bits 24-31, 6+6 bits
+    // per sibling); moving into such a sibling later then reads no table.
+    // An entry overwritten by a deeper level (or from an earlier round: LDS
+    // does not persist) fails its tag and the move reads the tables as
+    // before.
     uint32_t *sibc = nullptr;
     if constexpr (HEUR == 1) {
         sibc = (uint32_t *)(lds + (size_t)MAXD * sstride * sizeof(ST) + (size_t)blockDim.x * kLdsBoardBytes) +
@@ -423,15 +423,24 @@ __global__ void __launch_bounds__(256, (W == 4 ? 8 : 1)) k_round(KParams p, cons
             kv = (((down >= 2) ? dval2 : dval) >> (16 * (down & 1))) & 0xffffu;
             if constexpr (HEUR == 1) {
                 if (R) {
-                    // values of the next two passing children, 6 + 6 bits each
-                    const int s1 = __builtin_ctz(R);
-                    const uint32_t R2 = R & (R - 1);
-                    const int s2 = R2 ? __builtin_ctz(R2) : s1;
-                    const uint32_t v1 = (((s1 >= 2) ? dval2 : dval) >> (16 * (s1 & 1))) & 0xffffu;
-                    const uint32_t v2 = (((s2 >= 2) ? dval2 : dval) >> (16 * (s2 & 1))) & 0xffffu;
-                    const uint32_t p1 = (v1 & 63u) | (((v1 >> 8) & 63u) << 6);
-                    const uint32_t p2 = (v2 & 63u) | (((v2 >> 8) & 63u) << 6);
-                    sibc[(slot % kSibLevels) * blockDim.x] = ((uint32_t)slot << 24) | p1 | (p2 << 12);
+                    // values of the next two passing children, 6 + 6 bits
+                    // each.  An entry holds two siblings: a node with three
+                    // more (an item rooted at an instance's root, all four
+                    // moves passing) voids its level's entry instead, so
+                    // that none of them takes values left by an older node
+                    // of this depth, or the zeros shifted in after two
+                    uint32_t ent = 0xffffffffu;
+                    if (__builtin_popcount(R) <= 2) {
+                        const int s1 = __builtin_ctz(R);
+                        const uint32_t R2 = R & (R - 1);
+                        const int s2 = R2 ? __builtin_ctz(R2) : s1;
+                        const uint32_t v1 = (((s1 >= 2) ? dval2 : dval) >> (16 * (s1 & 1))) & 0xffffu;
+                        const uint32_t v2 = (((s2 >= 2) ? dval2 : dval) >> (16 * (s2 & 1))) & 0xffffu;
+                        const uint32_t p1 = (v1 & 63u) | (((v1 >> 8) & 63u) << 6);
+                        const uint32_t p2 = (v2 & 63u) | (((v2 >> 8) & 63u) << 6);
+                        ent = ((uint32_t)slot << 24) | p1 | (p2 << 12);
+                    }
+                    sibc[(slot % kSibLevels) * blockDim.x] = ent;
                 }
             }
         }

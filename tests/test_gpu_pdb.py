@@ -82,6 +82,31 @@ def test_pdb_first_solution_lengths(pdb_solver, ref_sets, ref_results):
         assert r.opt_len == row["len"], i
 
 
+def test_pdb_root_item_with_four_children(pdb_solver):
+    """The root as the only frontier item: it can have four passing
+    children, one more than a lane's sibling-value cache holds.  Short walks
+    with the blank away from the edges, long rounds, few lanes (one lane
+    then visits all four children in one round): every per-iteration count
+    equals the oracle's."""
+    boards = []
+    for seed in range(6_100_000, 6_100_400):
+        b = O.walk(5, 8 + seed % 17, seed)
+        r, c = divmod(int(np.argmax(np.asarray(b) == 0)), 5)
+        if 0 < r < 4 and 0 < c < 4:
+            boards.append(b)
+        if len(boards) == 48:
+            break
+    expect = [O.ida(b, heuristic="pdb") for b in boards]
+    for kw in (dict(lanes=256, workgroup_size=64), dict()):
+        say("pdb root items", kw)
+        with spz.Solver(width=5, heuristic=spz.H_PDB666, frontier_target=1, round_budget=4096,
+                        round_time_us=-1, **kw) as s:
+            s.build_pdb()
+            res = s.solve(boards)
+        for k, (r, (opt, thr, cnt)) in enumerate(zip(res, expect)):
+            assert (r.opt_len, r.thresholds, r.counts) == (opt, thr, cnt), k
+
+
 def test_pdb_schedule_invariance(pdb_solver, ref_sets, ref_results):
     rows = ref_results["pdb_expand_nodes"]["rows"]
     idx = [i for i, r in enumerate(rows) if r["nodes"] < 5_000_000][:12]
