@@ -269,6 +269,8 @@ constexpr int kLanesPerCu5 = 768;
 // less finely (a single 15-puzzle instance, 2048 frontier nodes: 65536 lanes
 // ran 2.4x as fast as 524288, and faster than 32768 or 131072).
 constexpr size_t kLanesPerItem = 32;
+// step budget of the rounds that run while the queue holds little work
+constexpr int kShortRoundSteps = 128;
 
 // Lane capacity (from occupancy unless given) and lane-context arrays for
 // the context's workgroup size and LDS stack depth.
@@ -1043,7 +1045,12 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
         p.round_ticks = (rt > 0 && ctx->wall_ticks_per_us > 0)
                             ? (unsigned long long)(rt * ctx->wall_ticks_per_us) : 0ull;
         ctl[9] = (unsigned)big;
-        ctl[10] = (unsigned)std::min(big, 32);
+        // rounds while the queue cannot keep half the lanes busy: short, so
+        // that donations refill idle lanes soon (measured on MI355X, 32 / 64
+        // / 128 / 256 / 512 steps: 100 40-move 15-puzzles 5270 / 6300 /
+        // 7270 / 7220 / 6040 instances/s, one random 15-puzzle per launch
+        // 44.7 / 45.2 / 46.5 / 45.1 / 39.4, the PDB search level)
+        ctl[10] = (unsigned)std::min(big, kShortRoundSteps);
         ctl[5] = ctl[10];
         HIPCHK(hipMemcpyAsync(ctx->ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemsetAsync(ctx->q_ctl.p, 0, 2 * kQCtl * 8, s));
