@@ -108,7 +108,7 @@ typedef struct spz_config {
                                 that step faster under load do not wait for
                                 the slowest; 0 = auto (4500 for the PDB search;
                                 Manhattan 3500 when the batch has work for
-                                every lane, else 2500), < 0 = off */
+                                every lane, else 700), < 0 = off */
     int32_t dedup;           /* 1: duplicate pruning in the root split: a
                                 top-tree node whose board an earlier kept node
                                 of its instance holds with a path-max f no

@@ -43,7 +43,7 @@ static void usage() {
                  "                         not the reference's counts)\n"
                  "  --round-time US        a search round also ends after US microseconds (-1: off;\n"
                  "                         default 4500 for the PDB search; Manhattan 3500 when the\n"
-                 "                         batch has work for every lane, else 2500)\n");
+                 "                         batch has work for every lane, else 700)\n");
 }
 
 int main(int argc, char **argv) {
