@@ -641,7 +641,10 @@ int orc_ida(int width, const uint8_t *board, int heuristic, int mode,
  * level with a smaller (path-max f, previous move)) holds its board with a
  * path-max f no larger: the closed list of the reference's A* root
  * distribution (src_horie/idas_smem.cu:1057-1149), whose counts no
- * reference file holds. */
+ * reference file holds.  Pinned for Manhattan by an 8-puzzle brute force
+ * (a consistent heuristic keeps exactly the BFS layers); under the PDB max,
+ * which is not consistent, parity unpinned beyond the properties (lengths
+ * kept, no count above the unpruned one). */
 typedef struct {
     uint8_t b[25];
     uint8_t g, h, F, prev; /* prev: 0..3, 4 = none */

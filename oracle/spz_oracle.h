@@ -77,7 +77,9 @@ int orc_ida(int width, const uint8_t *board, int heuristic, int mode,
  * whose board a kept node -- the root, a shallower node, or a same-level copy
  * with smaller (path-max f, previous move) -- holds with a path-max f no larger
  * is dropped before its level is expanded), the closed list of
- * src_horie/idas_smem.cu:1057-1149.
+ * src_horie/idas_smem.cu:1057-1149.  Pruned counts: pinned for Manhattan
+ * (8-puzzle BFS brute force); parity unpinned under the PDB heuristic beyond
+ * kept lengths and counts no larger than the unpruned ones.
  */
 int orc_ida_split(int width, const uint8_t *board, int heuristic, int target, int max_depth,
                   int dedup, int max_iters, uint64_t *counts, int *thresholds, int *n_iters);

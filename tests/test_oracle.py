@@ -398,3 +398,97 @@ def test_split_restatement_equals_plain_ida():
             assert all(x <= y for x, y in zip(cnt, exact[2]))
         # a shallow depth cap freezes the split early: same counts
         assert O.ida_split(b, target=5000, max_depth=3, max_iters=20) == exact
+
+
+def pruned_split_brute_force(board, target, max_depth=24):
+    """Duplicate-pruned split counts of an 8-puzzle from first principles.
+    Manhattan is consistent, so a node's path-max f is g + h and a board
+    kept at depth d covers every deeper copy: the kept level d is exactly
+    BFS layer d of the puzzle graph, each board once, with previous move the
+    smallest direction from a parent in layer d-1.  The frontier is the first
+    layer of at least `target` boards (or layer max_depth); iteration L counts
+    the children of every top-layer board with d + h <= L plus the plain
+    IDA* tree below every frontier board with D + h <= L.  Returns (optimal
+    length, thresholds, counts) for thresholds h0, h0+2, ... up to the
+    optimum (the BFS distance of the goal)."""
+    w = 3
+    root = tuple(int(x) for x in board)
+    goal = tuple(range(9))
+
+    def moved(b, d):
+        z = b.index(0)
+        nb = nbr(w, z, d)
+        if nb < 0:
+            return None
+        c = list(b)
+        c[z], c[nb] = c[nb], 0
+        return tuple(c)
+
+    layers, dist = [[root]], {root: 0}
+    D = opt = None
+    while D is None or opt is None:
+        d = len(layers) - 1
+        if D is None and (len(layers[d]) >= target or d >= max_depth):
+            D = d
+        if opt is None and goal in dist:
+            opt = dist[goal]
+        if D is not None and opt is not None:
+            break
+        nxt = []
+        for b in layers[d]:
+            for m in range(4):
+                c = moved(b, m)
+                if c is not None and c not in dist:
+                    dist[c] = d + 1
+                    nxt.append(c)
+        layers.append(nxt)
+
+    def prev_of(b, d):
+        # the kept copy's move: smallest direction m with a parent p in
+        # layer d-1 such that moving p's blank in direction m gives b
+        return min(m for m in range(4) if (p := moved(b, 3 - m)) is not None and dist.get(p) == d - 1)
+
+    def tree(b, g, prev, L):
+        h = md(w, b)
+        if g + h > L:
+            return 0
+        n = 0
+        for m in range(4):
+            if prev is not None and m == 3 - prev:
+                continue
+            c = moved(b, m)
+            if c is not None:
+                n += 1 + tree(c, g + 1, m, L)
+        return n
+
+    h0 = md(w, root)
+    thr, cnt = [], []
+    for L in range(h0, opt + 1, 2):
+        n = 0
+        for d in range(D):
+            for b in layers[d]:
+                if d + md(w, b) <= L:
+                    n += sum(moved(b, m) is not None for m in range(4)) - (1 if d else 0)
+        for b in layers[D]:
+            if D + md(w, b) <= L:
+                n += tree(b, D, prev_of(b, D) if D else None, L)
+        thr.append(L)
+        cnt.append(n)
+    return opt, thr, cnt
+
+
+def test_pruned_split_matches_bfs_brute_force():
+    """orc_ida_split with duplicate pruning against the brute force above on
+    8-puzzle walks, over frontier sizes that stop the split at depths 0 to
+    about 11 and a depth cap: a missed duplicate, a copy kept by the wrong
+    previous move or a count taken before pruning each changes a count."""
+    checked = 0
+    for i in range(8):
+        b = O.walk(3, 12 + 2 * i, 8_300_000 + i)
+        for target in (1, 3, 10, 40, 150, 600, 2000):
+            want = pruned_split_brute_force(b, target)
+            assert O.ida_split(b, target=target, dedup=True, max_iters=30) == want, (i, target)
+            checked += 1
+        want = pruned_split_brute_force(b, 10_000, max_depth=5)
+        assert O.ida_split(b, target=10_000, max_depth=5, dedup=True, max_iters=30) == want, i
+    assert checked == 56
