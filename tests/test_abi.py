@@ -111,3 +111,19 @@ def test_cli_usage():
     assert r.returncode == 0 and "usage: spz_solve" in r.stderr
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 2
+
+
+def test_cli_argument_errors(tmp_path):
+    """Errors the CLI reports before it touches a device: an unknown --lb
+    policy, --lb learned without parameters, and a malformed problem file."""
+    exe = os.path.join(REPO, "sliding-puzzle_amd", "bin", "spz_solve")
+    p = tmp_path / "prob"
+    p.write_text(" ".join(str((i + 1) % 16) for i in range(16)) + "\n")
+    r = subprocess.run([exe, "--lb", "fastest", str(p)], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage: spz_solve" in r.stderr
+    r = subprocess.run([exe, "--lb", "learned", str(p)], capture_output=True, text=True)
+    assert r.returncode == 2 and "--mlp-params" in r.stderr
+    bad = tmp_path / "bad"
+    bad.write_text("1 2 3\n")
+    r = subprocess.run([exe, str(bad)], capture_output=True, text=True)
+    assert r.returncode == 1 and "not a 4x4 or 5x5 board" in r.stderr

@@ -167,3 +167,38 @@ def test_cli_paths(tmp_path, golden):
             b[z], b[t] = b[t], 0
             z = t
         assert b == list(range(16))
+
+
+def test_cli_learned_lb(tmp_path, golden):
+    """--lb learned with --mlp-params (raw float32 parameters, as
+    spz_mlp_get_params returns them) changes the schedule only: the result
+    lines equal those of the default policy, with and without re-split; a
+    file of the wrong size is refused."""
+    fx = golden("iterations_korf100.json")["instances"][:5]
+    paths = []
+    for rec in fx:
+        p = tmp_path / f"prob{rec['idx']:03d}"
+        p.write_text(" ".join(map(str, rec["board"])) + " \n")
+        paths.append(str(p))
+    with spz.Solver(width=4) as s:
+        s.mlp_init(seed=5)
+        prm = np.asarray(s.mlp_get_params(), dtype=np.float32)
+    assert prm.size == spz.Solver.mlp_param_count()
+    pf = tmp_path / "mlp.f32"
+    prm.tofile(str(pf))
+    exe = os.path.join(REPO, "sliding-puzzle_amd", "bin", "spz_solve")
+    say("cli learned: uniform run")
+    base = subprocess.run([exe, "--batch", *paths], capture_output=True, text=True, timeout=120)
+    assert base.returncode == 0, base.stderr
+    assert len(base.stdout.splitlines()) == len(fx)
+    for extra in ([], ["--resplit", "2"]):
+        say("cli learned:", extra)
+        r = subprocess.run([exe, "--batch", "--lb", "learned", "--mlp-params", str(pf), *extra, *paths],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert r.stdout.splitlines() == base.stdout.splitlines()
+    short = tmp_path / "short.f32"
+    prm[:100].tofile(str(short))
+    r = subprocess.run([exe, "--batch", "--lb", "learned", "--mlp-params", str(short), *paths],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "float32 parameters" in r.stderr
