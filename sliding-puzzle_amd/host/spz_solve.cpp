@@ -10,7 +10,9 @@
 #include "spz.h"
 #include "spz_problem.h"
 
+#include <cerrno>
 #include <chrono>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -63,11 +65,38 @@ int main(int argc, char **argv) {
             }
             return argv[++i];
         };
+        // a whole decimal int, else the usage text and exit code 2
+        auto num = [&]() -> int {
+            const char *v = next();
+            char *end = nullptr;
+            errno = 0;
+            const long x = std::strtol(v, &end, 10);
+            if (end == v || *end != '\0' || errno == ERANGE || x < INT_MIN || x > INT_MAX) {
+                usage();
+                std::exit(2);
+            }
+            return (int)x;
+        };
         if (a == "-H") {
             std::string v = next();
-            heur = v == "pdb" ? SPZ_H_PDB666 : SPZ_H_MANHATTAN;
+            if (v == "pdb")
+                heur = SPZ_H_PDB666;
+            else if (v == "manhattan")
+                heur = SPZ_H_MANHATTAN;
+            else {
+                usage();
+                return 2;
+            }
         } else if (a == "-m") {
-            mode = std::string(next()) == "first" ? SPZ_MODE_FIRST : SPZ_MODE_ALL;
+            std::string v = next();
+            if (v == "first")
+                mode = SPZ_MODE_FIRST;
+            else if (v == "all")
+                mode = SPZ_MODE_ALL;
+            else {
+                usage();
+                return 2;
+            }
         } else if (a == "--batch") {
             batch = true;
         } else if (a == "--iters") {
@@ -85,7 +114,7 @@ int main(int argc, char **argv) {
         } else if (a == "--mlp-params") {
             mlpfile = next();
         } else if (a == "--resplit") {
-            resplit = std::atoi(next());
+            resplit = num();
         } else if (a == "--lb") {
             std::string v = next();
             if (v == "prevload")
@@ -99,17 +128,17 @@ int main(int argc, char **argv) {
                 return 2;
             }
         } else if (a == "--frontier") {
-            frontier = std::atoi(next());
+            frontier = num();
         } else if (a == "--wg") {
-            wg = std::atoi(next());
+            wg = num();
         } else if (a == "--lds-depth") {
-            depth = std::atoi(next());
+            depth = num();
         } else if (a == "--budget") {
-            budget = std::atoi(next());
+            budget = num();
         } else if (a == "--round-time") {
-            round_us = std::atoi(next());
+            round_us = num();
         } else if (a == "--max-iters") {
-            max_iters = std::atoi(next());
+            max_iters = num();
         } else if (a == "--dedup") {
             dedup = true;
         } else if (a == "--no-lookahead") {
@@ -126,7 +155,7 @@ int main(int argc, char **argv) {
             files.push_back(a);
         }
     }
-    if (files.empty() && tabdir.empty()) {
+    if ((files.empty() && tabdir.empty()) || max_iters < 1) {
         usage();
         return 2;
     }

@@ -121,6 +121,13 @@ def test_cli_argument_errors(tmp_path):
     p.write_text(" ".join(str((i + 1) % 16) for i in range(16)) + "\n")
     r = subprocess.run([exe, "--lb", "fastest", str(p)], capture_output=True, text=True)
     assert r.returncode == 2 and "usage: spz_solve" in r.stderr
+    # unknown heuristic or mode names and non-numeric or out-of-range sizes
+    # are refused, not read as the defaults
+    for args in (["-H", "pbd"], ["-m", "frist"], ["--wg", "abc"], ["--frontier", "12x"],
+                 ["--budget", ""], ["--max-iters", "99999999999"], ["--max-iters", "0"],
+                 ["--max-iters", "-3"], ["--round-time"]):
+        r = subprocess.run([exe, *args, str(p)], capture_output=True, text=True)
+        assert r.returncode == 2 and "usage: spz_solve" in r.stderr, args
     r = subprocess.run([exe, "--lb", "learned", str(p)], capture_output=True, text=True)
     assert r.returncode == 2 and "--mlp-params" in r.stderr
     bad = tmp_path / "bad"
