@@ -67,7 +67,7 @@ typedef struct spz_config {
                                 tree breadth-first until a level holds at
                                 least this many nodes (reference: 2048,
                                 src/15puzzle_block_parallel.cu:150-187);
-                                1 = no split */
+                                1 = no split; at most 1048576 */
     int32_t max_split_depth; /* cap on that split depth (0 = 24) */
     int32_t workgroup_size;  /* 64, 128 or 256 lanes (0 = 256) */
     int32_t lanes;           /* DFS lanes in flight (0 = fill the device) */

@@ -304,8 +304,15 @@ static int configure_lanes(spz_ctx *ctx) {
     return SPZ_OK;
 }
 
+// Largest root-split target: a level holds at most 4 x target items per
+// instance, so this keeps that capacity far inside 32-bit counts and the
+// frontier buffer at 128 MB per instance.
+constexpr int kMaxFrontierTarget = 1 << 20;
+
 static int normalise_config(const spz_config *in, spz_config *c, int *wg) {
     *c = *in;
+    if (c->mode != SPZ_MODE_ALL && c->mode != SPZ_MODE_FIRST) return SPZ_E_ARG;
+    if (c->frontier_target > kMaxFrontierTarget) return SPZ_E_ARG;
     *wg = c->workgroup_size ? c->workgroup_size : 256;
     if (*wg != 64 && *wg != 128 && *wg != 256) return SPZ_E_ARG;
     c->workgroup_size = *wg;
@@ -429,7 +436,8 @@ int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_ite
     spz_config base = ctx->cfg;
     const int T = base.frontier_target;
     const int wgs[3] = {64, 128, 256};
-    const int tgts[5] = {std::max(1, T / 16), std::max(1, T / 4), T, T * 2, T * 4};
+    const int tgts[5] = {std::max(1, T / 16), std::max(1, T / 4), T, std::min(T * 2, kMaxFrontierTarget),
+                         std::min(T * 4, kMaxFrontierTarget)};
     const int deps_w4[4] = {48, 64, 80, 112}, deps_w5[4] = {64, 80, 96, 128};
     const int *deps = base.width == 4 ? deps_w4 : deps_w5;
     const int buds[4] = {256, 512, 1024, 2048};

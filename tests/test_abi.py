@@ -81,8 +81,33 @@ def test_default_config_values():
     assert (cfg.width, cfg.heuristic, cfg.mode, cfg.frontier_target) == (5, 1, 0, 2048)
     assert cfg.workgroup_size == 256 and cfg.time_kernels == 0
     assert cfg.lookahead == 1 and cfg.record_paths == 0 and cfg.use_graph == 1
-    assert cfg.round_time_us == 0  # auto: 4500 us for the PDB search, 2500 for Manhattan
+    assert cfg.round_time_us == 0  # auto: 4500 us for the PDB search; Manhattan 3500, or 700 below full
     assert cfg.dedup == 0  # parity: the reference's counts
+
+
+def test_create_rejects_bad_configs():
+    """spz_create checks its configuration before it touches a device: a bad
+    one is SPZ_E_ARG here too, a good one fails only for want of a GPU."""
+    import ctypes
+
+    def create(**kw):
+        cfg = spz.Config()
+        spz.lib().spz_default_config(ctypes.byref(cfg), 4, spz.H_MANHATTAN)
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        ctx = ctypes.c_void_p()
+        rc = spz.lib().spz_create(ctypes.byref(ctx), ctypes.byref(cfg))
+        spz.lib().spz_destroy(ctx)
+        return rc
+
+    E_ARG = -1
+    for kw in (dict(mode=2), dict(mode=-1), dict(frontier_target=(1 << 20) + 1), dict(frontier_target=1 << 30),
+               dict(workgroup_size=96), dict(lds_stack_depth=4), dict(lb_policy=9), dict(resplit=-1),
+               dict(width=6), dict(heuristic=spz.H_PDB666)):
+        assert create(**kw) == E_ARG, kw
+    # valid: created on a GPU, SPZ_E_HIP without one
+    for kw in (dict(), dict(mode=1), dict(frontier_target=1 << 20)):
+        assert create(**kw) != E_ARG, kw
 
 
 def test_config_layout_matches_c(tmp_path):
