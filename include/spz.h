@@ -299,9 +299,11 @@ void spz_pdb_geometry(uint8_t *tiles48, uint8_t *maps200);
 /* Seeded instance generators (splitmix64-seeded xoshiro256**):
  * a non-backtracking walk of `steps` blank moves from the goal, or a
  * uniformly random solvable permutation.  The reference's generator
- * (src/make_24puzzle_problems.cc:54-77) backtracks; these do not. */
-void spz_gen_walk(int32_t width, int32_t steps, uint64_t seed, uint8_t *out);
-void spz_gen_random(int32_t width, uint64_t seed, uint8_t *out);
+ * (src/make_24puzzle_problems.cc:54-77) backtracks; these do not.  `out`
+ * takes width*width cells; SPZ_E_ARG (nothing written) for a null `out`, a
+ * width outside 2..5 or negative steps. */
+int32_t spz_gen_walk(int32_t width, int32_t steps, uint64_t seed, uint8_t *out);
+int32_t spz_gen_random(int32_t width, uint64_t seed, uint8_t *out);
 
 #ifdef __cplusplus
 }

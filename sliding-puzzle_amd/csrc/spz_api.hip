@@ -1580,7 +1580,8 @@ struct Xoshiro {
     int below(int k) { return (int)(((next() >> 32) * (uint64_t)k) >> 32); }
 };
 
-void spz_gen_walk(int32_t width, int32_t steps, uint64_t seed, uint8_t *out) {
+int32_t spz_gen_walk(int32_t width, int32_t steps, uint64_t seed, uint8_t *out) {
+    if (!out || width < 2 || width > 5 || steps < 0) return SPZ_E_ARG;
     int n2 = width * width, blank = 0, prev = -1;
     Xoshiro r(seed);
     for (int i = 0; i < n2; ++i) out[i] = (uint8_t)i;
@@ -1600,9 +1601,11 @@ void spz_gen_walk(int32_t width, int32_t steps, uint64_t seed, uint8_t *out) {
         blank = nb;
         prev = d;
     }
+    return SPZ_OK;
 }
 
-void spz_gen_random(int32_t width, uint64_t seed, uint8_t *out) {
+int32_t spz_gen_random(int32_t width, uint64_t seed, uint8_t *out) {
+    if (!out || width < 2 || width > 5) return SPZ_E_ARG;
     int n2 = width * width;
     Xoshiro r(seed);
     for (;;) {
@@ -1611,7 +1614,7 @@ void spz_gen_random(int32_t width, uint64_t seed, uint8_t *out) {
             int j = r.below(i + 1);
             std::swap(out[i], out[j]);
         }
-        if (is_solvable(width, out)) return;
+        if (is_solvable(width, out)) return SPZ_OK;
     }
 }
 

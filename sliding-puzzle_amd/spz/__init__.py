@@ -134,9 +134,9 @@ def lib() -> ctypes.CDLL:
     L.spz_pdb_geometry.argtypes = [u8p, u8p]
     L.spz_pdb_geometry.restype = None
     L.spz_gen_walk.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, u8p]
-    L.spz_gen_walk.restype = None
+    L.spz_gen_walk.restype = ctypes.c_int32
     L.spz_gen_random.argtypes = [ctypes.c_int32, ctypes.c_uint64, u8p]
-    L.spz_gen_random.restype = None
+    L.spz_gen_random.restype = ctypes.c_int32
     _lib = L
     return L
 
@@ -146,14 +146,16 @@ def _u8p(a: np.ndarray):
 
 
 def gen_walk(width: int, steps: int, seed: int) -> np.ndarray:
-    out = np.zeros(width * width, dtype=np.uint8)
-    lib().spz_gen_walk(width, steps, seed, _u8p(out))
+    out = np.zeros(max(width, 0) ** 2, dtype=np.uint8)
+    if lib().spz_gen_walk(width, steps, seed, _u8p(out)) != 0:
+        raise ValueError(f"spz_gen_walk: width {width} not in 2..5 or steps {steps} < 0")
     return out
 
 
 def gen_random(width: int, seed: int) -> np.ndarray:
-    out = np.zeros(width * width, dtype=np.uint8)
-    lib().spz_gen_random(width, seed, _u8p(out))
+    out = np.zeros(max(width, 0) ** 2, dtype=np.uint8)
+    if lib().spz_gen_random(width, seed, _u8p(out)) != 0:
+        raise ValueError(f"spz_gen_random: width {width} not in 2..5")
     return out
 
 

@@ -41,11 +41,29 @@ def test_header_compiles_as_c():
     assert r.returncode == 0, r.stderr
 
 
-@pytest.mark.parametrize("w", [4, 5])
+@pytest.mark.parametrize("w", [2, 3, 4, 5])
 def test_generators_match_oracle(w):
     for i in range(30):
         assert (spz.gen_walk(w, 40 + i, 1_000_000 + i) == O.walk(w, 40 + i, 1_000_000 + i)).all()
         assert (spz.gen_random(w, 2_000_000 + i) == O.random_solvable(w, 2_000_000 + i)).all()
+
+
+def test_generators_reject_bad_arguments():
+    """A width outside 2..5 (width 1 has no legal move; 0 would divide by
+    zero), negative steps or a null buffer: SPZ_E_ARG and nothing written."""
+    import ctypes
+    buf = np.full(64, 77, dtype=np.uint8)
+    p = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    L = spz.lib()
+    for w, steps in ((1, 5), (0, 5), (-2, 5), (6, 5), (4, -1)):
+        assert L.spz_gen_walk(w, steps, 1, p) == -1, (w, steps)
+    for w in (1, 0, 6):
+        assert L.spz_gen_random(w, 1, p) == -1, w
+    assert (buf == 77).all()
+    assert L.spz_gen_walk(4, 10, 1, None) == -1 and L.spz_gen_random(4, 1, None) == -1
+    assert L.spz_gen_walk(4, 0, 1, p) == 0 and list(buf[:16]) == list(range(16))
+    with pytest.raises(ValueError):
+        spz.gen_walk(7, 10, 1)
 
 
 def test_pdb_geometry_matches_oracle():
