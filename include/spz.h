@@ -286,8 +286,8 @@ void spz_destroy(spz_ctx *ctx);
  * the two direct tables (span_bytes <= 0: all 2 x 25^6 bytes) -- the
  * search's access pattern without the search.  Over the whole tables nearly
  * every gather misses L2; over a span of a few MB nearly every one hits:
- * the two measured random-access ceilings of the PDB kernel.  No reference
- * counterpart. */
+ * the two measured random-access ceilings of the PDB kernel.  n_gathers in
+ * 1..2^36.  No reference counterpart. */
 int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, int64_t span_bytes, double *ms, double *gathers_per_s);
 
 /* Diagnostics: the eight PDB lookups as (tiles in slot order, cell map),

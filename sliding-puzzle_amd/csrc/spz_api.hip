@@ -1313,6 +1313,8 @@ int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens) 
 
 int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, int64_t span_bytes, double *ms, double *gathers_per_s) {
     if (!ctx || !ms || !gathers_per_s || n_gathers <= 0) return SPZ_E_ARG;
+    // the grid is n_gathers / 2^16 workgroups of 256 threads: keep it an int
+    if (n_gathers > (1ll << 36)) return fail(ctx, SPZ_E_ARG, "n_gathers above 2^36");
     if (!ctx->have_pdb) return fail(ctx, SPZ_E_NOPDB, "call spz_build_pdb first");
     const uint64_t both = 2ull * kPdbDirect; // the two direct tables, end to end
     const uint64_t span = (span_bytes <= 0 || (uint64_t)span_bytes > both) ? both : (uint64_t)span_bytes;
