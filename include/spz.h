@@ -36,6 +36,8 @@ extern "C" {
 #define SPZ_E_STALL -7    /* internal: work vanished with instances active */
 #define SPZ_E_TIMEOUT -8  /* wall-clock limit (env SPZ_TIMEOUT_S, default 900) */
 #define SPZ_E_MISMATCH -9 /* spz_autotune: a candidate configuration changed the result */
+#define SPZ_E_HOST -10    /* host-side failure (out of host memory, a thread that cannot
+                             start); no C++ exception leaves the library */
 
 /* heuristics */
 #define SPZ_H_MANHATTAN 0 /* src/24puzzle.cc:89-108 */

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <exception>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -175,6 +176,25 @@ static int fail(spz_ctx *c, int code, const char *fmt, ...) {
     return code;
 }
 
+// No exception may leave a C entry point: a host-side failure (out of host
+// memory, a thread that cannot start) returns SPZ_E_HOST with its text.
+static int host_failure(spz_ctx *c, const char *what) noexcept {
+    if (c) {
+        try {
+            c->err = std::string("host: ") + what;
+        } catch (...) {
+        }
+    }
+    return SPZ_E_HOST;
+}
+#define SPZ_CATCH(c)                                                                                       \
+    catch (const std::exception &ex) {                                                                     \
+        return host_failure((c), ex.what());                                                              \
+    }                                                                                                      \
+    catch (...) {                                                                                          \
+        return host_failure((c), "unknown exception");                                                    \
+    }
+
 #define HIPCHK(expr)                                                                                       \
     do {                                                                                                   \
         hipError_t e_ = (expr);                                                                            \
@@ -326,7 +346,7 @@ static int normalise_config(const spz_config *in, spz_config *c, int *wg) {
     return SPZ_OK;
 }
 
-int spz_create(spz_ctx **out, const spz_config *cfg) {
+int spz_create(spz_ctx **out, const spz_config *cfg) try {
     spz_ctx *ctx = nullptr;
     if (!out || !cfg) return SPZ_E_ARG;
     *out = nullptr;
@@ -394,9 +414,9 @@ int spz_create(spz_ctx **out, const spz_config *cfg) {
     }
     *out = ctx;
     return SPZ_OK;
-}
+} SPZ_CATCH(nullptr)
 
-int spz_reconfigure(spz_ctx *ctx, const spz_config *cfg) {
+int spz_reconfigure(spz_ctx *ctx, const spz_config *cfg) try {
     if (!ctx || !cfg) return SPZ_E_ARG;
     if (cfg->width != ctx->cfg.width || cfg->heuristic != ctx->cfg.heuristic || cfg->device != ctx->cfg.device)
         return fail(ctx, SPZ_E_ARG, "width, heuristic and device are fixed for a context");
@@ -407,7 +427,7 @@ int spz_reconfigure(spz_ctx *ctx, const spz_config *cfg) {
     ctx->cfg = ncfg;
     ctx->wg = wg;
     return configure_lanes(ctx);
-}
+} SPZ_CATCH(ctx)
 
 void spz_get_config(spz_ctx *ctx, spz_config *cfg) {
     if (ctx && cfg) *cfg = ctx->cfg;
@@ -427,7 +447,7 @@ void spz_get_config(spz_ctx *ctx, spz_config *cfg) {
 //  - duplicate pruning: the pruned counts depend on the root-split depth, so
 //    the frontier target is not varied (the other dimensions keep counts).
 int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_iters, int32_t passes,
-                 spz_config *best, double *best_ms, int32_t *tried) {
+                 spz_config *best, double *best_ms, int32_t *tried) try {
     if (!ctx || !boards || n <= 0 || !best) return SPZ_E_ARG;
     if (max_iters < 1 || max_iters > kMaxIters) return fail(ctx, SPZ_E_ARG, "max_iters must be in 1..%d", kMaxIters);
     if (passes <= 0) passes = 1;
@@ -513,7 +533,7 @@ int spz_autotune(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_ite
     if (best_ms) *best_ms = bms;
     if (tried) *tried = count;
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 const char *spz_last_error(spz_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
@@ -542,19 +562,19 @@ void spz_destroy(spz_ctx *ctx) {
     delete ctx;
 }
 
-int spz_last_stats(spz_ctx *ctx, double *stats) {
+int spz_last_stats(spz_ctx *ctx, double *stats) try {
     if (!ctx || !stats) return SPZ_E_ARG;
     memcpy(stats, ctx->stats, sizeof ctx->stats);
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
-int spz_set_lb(spz_ctx *ctx, int32_t lb_policy) {
+int spz_set_lb(spz_ctx *ctx, int32_t lb_policy) try {
     if (!ctx) return SPZ_E_ARG;
     if (lb_policy != SPZ_LB_UNIFORM && lb_policy != SPZ_LB_LEARNED && lb_policy != SPZ_LB_PREVLOAD)
         return fail(ctx, SPZ_E_ARG, "unknown lb policy %d", lb_policy);
     ctx->cfg.lb_policy = lb_policy;
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 // ---------------------------------------------------------------- PDB ---
 static int upload_direct(spz_ctx *ctx) {
@@ -645,7 +665,7 @@ static int write_cache(spz_ctx *ctx, const char *path) {
     return SPZ_OK;
 }
 
-int spz_build_pdb(spz_ctx *ctx, const char *cache_path) {
+int spz_build_pdb(spz_ctx *ctx, const char *cache_path) try {
     if (!ctx) return SPZ_E_ARG;
     if (ctx->cfg.width != 5) return fail(ctx, SPZ_E_ARG, "PDBs exist for the 24-puzzle only");
     HIPCHK(hipSetDevice(ctx->cfg.device));
@@ -679,13 +699,13 @@ int spz_build_pdb(spz_ctx *ctx, const char *cache_path) {
     if (rc != SPZ_OK) return rc;
     if (cache_path) return write_cache(ctx, cache_path);
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 // The reference's table files (src/24puzzle_with_pdb.cc:77-112): each is
 // exactly 127,512,000 bytes, one byte per placement of the pattern's six
 // tiles in nested-loop order.  Besides the size, the entry of the goal
 // placement (every pattern tile home) must be 0.
-int spz_load_pdb_tab(spz_ctx *ctx, const char *path_1256712, const char *path_34891314) {
+int spz_load_pdb_tab(spz_ctx *ctx, const char *path_1256712, const char *path_34891314) try {
     if (!ctx || !path_1256712 || !path_34891314) return SPZ_E_ARG;
     if (ctx->cfg.width != 5) return fail(ctx, SPZ_E_ARG, "PDBs exist for the 24-puzzle only");
     HIPCHK(hipSetDevice(ctx->cfg.device));
@@ -705,16 +725,16 @@ int spz_load_pdb_tab(spz_ctx *ctx, const char *path_1256712, const char *path_34
     }
     ctx->have_pdb = false;
     return upload_compact(ctx, buf.data(), buf.data() + kPlacementsHost);
-}
+} SPZ_CATCH(ctx)
 
-int spz_get_pdb(spz_ctx *ctx, int32_t pattern, uint8_t *out) {
+int spz_get_pdb(spz_ctx *ctx, int32_t pattern, uint8_t *out) try {
     if (!ctx || !out || pattern < 0 || pattern > 1) return SPZ_E_ARG;
     if (!ctx->have_pdb) return fail(ctx, SPZ_E_NOPDB, "no PDB built");
     HIPCHK(hipMemcpy(out, ctx->pdb_compact[pattern].p, kPlacementsHost, hipMemcpyDeviceToHost));
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
-int spz_export_pdb_tab(spz_ctx *ctx, const char *p0, const char *p1) {
+int spz_export_pdb_tab(spz_ctx *ctx, const char *p0, const char *p1) try {
     if (!ctx || !p0 || !p1) return SPZ_E_ARG;
     if (!ctx->have_pdb) return fail(ctx, SPZ_E_NOPDB, "no PDB built");
     std::vector<uint8_t> buf(kPlacementsHost);
@@ -728,7 +748,7 @@ int spz_export_pdb_tab(spz_ctx *ctx, const char *p0, const char *p1) {
         if (w != buf.size()) return fail(ctx, SPZ_E_IO, "short write to %s", paths[t]);
     }
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 // -------------------------------------------------------------- solve ---
 // Duplicate pruning in the root split (config dedup).  Before a level is
@@ -811,7 +831,7 @@ static size_t next_pow2(size_t x) {
     return p;
 }
 
-int spz_eval_h(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h) {
+int spz_eval_h(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h) try {
     if (!ctx || !boards || !h || n < 0) return SPZ_E_ARG;
     if (n == 0) return SPZ_OK;
     const int w = ctx->cfg.width, heur = ctx->cfg.heuristic;
@@ -834,12 +854,12 @@ int spz_eval_h(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h) {
     if (e != hipSuccess) return fail(ctx, SPZ_E_HIP, "eval_h: %s", hipGetErrorString(e));
     for (int i = 0; i < n; ++i) h[i] = meta_h(items[i].meta);
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 // spz_eval_h without any allocation: boards and values pass through the
 // device's `scratch` array (every call on a GPU shares it; calls that
 // overlap on one GPU race), the kernel reads the tables the context holds.
-int spz_eval_h_noalloc(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h) {
+int spz_eval_h_noalloc(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *h) try {
     if (!ctx || !boards || !h || n < 0) return SPZ_E_ARG;
     if (n == 0) return SPZ_OK;
     if (n > kNoAllocMaxBoards)
@@ -859,10 +879,10 @@ int spz_eval_h_noalloc(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t *
     HIPCHK(hipMemcpyAsync(h, hd, (size_t)n * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_iters, spz_result *out,
-                    uint64_t *iter_counts) {
+                    uint64_t *iter_counts) try {
     if (!ctx || !boards || !out || n < 0) return SPZ_E_ARG;
     if (max_iters < 1 || max_iters > kMaxIters)
         return fail(ctx, SPZ_E_ARG, "max_iters must be in 1..%d", kMaxIters);
@@ -1251,7 +1271,7 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     }
     ctx->stats[12] = all_nodes;
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur, bool paths,
                               std::vector<SeenBoards> &seen, hipStream_t s) {
@@ -1291,7 +1311,7 @@ static hipError_t dedup_level(spz_ctx *ctx, int n, int capl, int width, int cur,
     return e;
 }
 
-int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens) {
+int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens) try {
     if (!ctx || !lens || max_len < 0 || (max_len > 0 && !moves)) return SPZ_E_ARG;
     if (ctx->paths_n <= 0) return fail(ctx, SPZ_E_ARG, "the last solve recorded no paths (config record_paths)");
     HIPCHK(hipSetDevice(ctx->cfg.device));
@@ -1309,9 +1329,9 @@ int spz_get_paths(spz_ctx *ctx, uint8_t *moves, int32_t max_len, int32_t *lens) 
             moves[i * max_len + j] = j < len[sl] ? (uint8_t)((w[sl * kPathWords + j / 16] >> (2 * (j % 16))) & 3u) : 0;
     }
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
-int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, int64_t span_bytes, double *ms, double *gathers_per_s) {
+int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, int64_t span_bytes, double *ms, double *gathers_per_s) try {
     if (!ctx || !ms || !gathers_per_s || n_gathers <= 0) return SPZ_E_ARG;
     // the grid is n_gathers / 2^16 workgroups of 256 threads: keep it an int
     if (n_gathers > (1ll << 36)) return fail(ctx, SPZ_E_ARG, "n_gathers above 2^36");
@@ -1344,7 +1364,7 @@ int spz_gather_probe(spz_ctx *ctx, int64_t n_gathers, int64_t span_bytes, double
     *ms = t;
     *gathers_per_s = (double)blocks * 256.0 * per_thread / (t * 1e-3);
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 // ------------------------------------------------------------ learned LB ---
 int32_t spz_mlp_param_count(void) { return (int32_t)kMlpParams; }
@@ -1357,30 +1377,30 @@ static int mlp_ready_alloc(spz_ctx *ctx) {
     return SPZ_OK;
 }
 
-int spz_mlp_init(spz_ctx *ctx, uint64_t seed) {
+int spz_mlp_init(spz_ctx *ctx, uint64_t seed) try {
     if (!ctx) return SPZ_E_ARG;
     HIPCHK(hipSetDevice(ctx->cfg.device));
     int rc = mlp_ready_alloc(ctx);
     if (rc) return rc;
     HIPCHK(mlp_init_random(ctx->mlp, seed, ctx->stream));
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
-int spz_mlp_set_params(spz_ctx *ctx, const float *params) {
+int spz_mlp_set_params(spz_ctx *ctx, const float *params) try {
     if (!ctx || !params) return SPZ_E_ARG;
     HIPCHK(hipSetDevice(ctx->cfg.device));
     int rc = mlp_ready_alloc(ctx);
     if (rc) return rc;
     HIPCHK(mlp_set_params(ctx->mlp, params, ctx->stream));
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
-int spz_mlp_get_params(spz_ctx *ctx, float *params) {
+int spz_mlp_get_params(spz_ctx *ctx, float *params) try {
     if (!ctx || !params) return SPZ_E_ARG;
     if (!ctx->mlp_alloced || !ctx->mlp.ready) return fail(ctx, SPZ_E_ARG, "no MLP parameters");
     HIPCHK(mlp_get_params(ctx->mlp, params, ctx->stream));
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 // boards -> Items with the context's heuristic in meta -> feature rows
 static int mlp_load_rows(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, int32_t n, DBuf &items,
@@ -1403,7 +1423,7 @@ static int mlp_load_rows(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr
     return SPZ_OK;
 }
 
-int spz_mlp_forward(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, int32_t n, float *out) {
+int spz_mlp_forward(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, int32_t n, float *out) try {
     if (!ctx || !boards || !thr || !out || n <= 0) return SPZ_E_ARG;
     if (!ctx->mlp_alloced || !ctx->mlp.ready) return fail(ctx, SPZ_E_ARG, "no MLP parameters");
     HIPCHK(hipSetDevice(ctx->cfg.device));
@@ -1414,11 +1434,11 @@ int spz_mlp_forward(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, int
     HIPCHK(hipMemcpyAsync(out, ctx->mlp.y, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 static_assert(kPredK == kPredThr, "one prediction per threshold slot of the restart");
 
-int spz_mlp_predict(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr0, int32_t n, float *out) {
+int spz_mlp_predict(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr0, int32_t n, float *out) try {
     if (!ctx || !boards || !thr0 || !out || n <= 0) return SPZ_E_ARG;
     if (!ctx->mlp_alloced || !ctx->mlp.ready) return fail(ctx, SPZ_E_ARG, "no MLP parameters");
     HIPCHK(hipSetDevice(ctx->cfg.device));
@@ -1431,10 +1451,10 @@ int spz_mlp_predict(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr0, in
     HIPCHK(hipMemcpyAsync(out, y.p, (size_t)n * kPredThr * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 int spz_mlp_grad(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const float *target, int32_t n,
-                 float *loss, float *grad) {
+                 float *loss, float *grad) try {
     if (!ctx || !boards || !thr || !target || !loss || !grad || n <= 0) return SPZ_E_ARG;
     if (!ctx->mlp_alloced || !ctx->mlp.ready) return fail(ctx, SPZ_E_ARG, "no MLP parameters");
     HIPCHK(hipSetDevice(ctx->cfg.device));
@@ -1449,10 +1469,10 @@ int spz_mlp_grad(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const 
     HIPCHK(hipMemcpyAsync(grad, ctx->mlp.grad, kMlpParams * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 int spz_mlp_train(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const float *target, int32_t n,
-                  int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss) {
+                  int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss) try {
     if (!ctx || !boards || !thr || !target || n <= 0 || epochs <= 0 || batch <= 0) return SPZ_E_ARG;
     if (!ctx->mlp_alloced || !ctx->mlp.ready) return fail(ctx, SPZ_E_ARG, "initialise the MLP first");
     HIPCHK(hipSetDevice(ctx->cfg.device));
@@ -1513,9 +1533,9 @@ int spz_mlp_train(spz_ctx *ctx, const uint8_t *boards, const int32_t *thr, const
     if (epoch_loss)
         for (int ep = 0; ep < epochs; ++ep) epoch_loss[ep] = nbatches[ep] ? sums[ep] / (float)nbatches[ep] : 0.f;
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
-int spz_get_samples(spz_ctx *ctx, uint8_t *boards, int32_t *thr, uint64_t *loads, int32_t max, int32_t *n_out) {
+int spz_get_samples(spz_ctx *ctx, uint8_t *boards, int32_t *thr, uint64_t *loads, int32_t max, int32_t *n_out) try {
     if (!ctx || !n_out) return SPZ_E_ARG;
     *n_out = 0;
     if (!ctx->samples.p || !ctx->nsamp.p) return SPZ_OK;
@@ -1545,7 +1565,7 @@ int spz_get_samples(spz_ctx *ctx, uint8_t *boards, int32_t *thr, uint64_t *loads
     }
     *n_out = (int32_t)n;
     return SPZ_OK;
-}
+} SPZ_CATCH(ctx)
 
 void spz_pdb_geometry(uint8_t *tiles48, uint8_t *maps200) {
     PdbGeo g;

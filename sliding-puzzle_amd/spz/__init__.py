@@ -34,7 +34,7 @@ MAX_ITERS = 128
 
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "no PDB", -4: "queue capacity",
           -5: "I/O error", -6: "unsolvable", -7: "stalled", -8: "timeout",
-          -9: "autotune result mismatch"}
+          -9: "autotune result mismatch", -10: "host failure"}
 
 
 class SpzError(RuntimeError):
