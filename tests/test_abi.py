@@ -34,6 +34,14 @@ def test_library_exports_every_declared_symbol():
     assert set(spz.EXPORTS) == set(names)
 
 
+def test_error_codes_match_header():
+    """Every SPZ_E_* code of include/spz.h has a name in the Python mirror's
+    error table, and the table names no other code."""
+    codes = {int(v) for v in re.findall(r"#define SPZ_E_[A-Z_]+\s+(-\d+)", open(HEADER).read())}
+    assert len(codes) >= 10
+    assert codes == set(spz.ERRORS)
+
+
 def test_header_compiles_as_c():
     src = '#include "spz.h"\nint main(void){spz_config c; spz_default_config(&c,4,0); return c.width!=4;}\n'
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(REPO, "include"),
