@@ -351,6 +351,9 @@ def main():
         print(json.dumps({"error": "this solver runs on one GPU; --gpus above 1 is not supported "
                                    "(instance batches shard across GPUs with no exchange; not built here)"}))
         sys.exit(2)
+    if args.steps < 1 or args.warmup < 0 or args.config not in FRONTIER:
+        print(json.dumps({"error": f"need --steps >= 1, --warmup >= 0 and --config one of {sorted(FRONTIER)}"}))
+        sys.exit(2)
 
     if args.copy_probe:
         import torch
