@@ -359,6 +359,20 @@ def test_pdb_tables_checksums_and_invariants(pdb_tables, golden):
             assert t[r] >= m and (t[r] - m) % 2 == 0
 
 
+def test_pdb_heuristic_is_not_consistent(pdb_tables):
+    """The 6-6-6-6 PDB heuristic (the larger of two additive partitions,
+    each entry the fewest pattern-tile moves over every blank position) can
+    change by more than 1 in one move, so a node's path-max f is not its
+    g + h and the BFS-layer check of duplicate pruning covers Manhattan
+    only.  A board where sliding one tile raises h by 3:"""
+    b = np.array([5, 1, 2, 3, 4, 10, 6, 7, 12, 9, 15, 11, 13, 8, 14, 16, 21, 17, 0, 19, 20, 22, 23, 18, 24],
+                 dtype=np.uint8)
+    c = b.copy()
+    c[18], c[13] = c[13], 0  # the blank (cell 18) moves up to cell 13
+    assert O.pdb_h(c) - O.pdb_h(b) == 3
+    assert abs(O.manhattan(c) - O.manhattan(b)) == 1
+
+
 def test_pdb_expand_nodes_csv(pdb_tables, ref_sets, ref_results):
     rows = ref_results["pdb_expand_nodes"]["rows"]
     for i, row in enumerate(rows):
