@@ -38,7 +38,11 @@ ERRORS = {-1: "bad argument", -2: "HIP error", -3: "no PDB", -4: "queue capacity
 
 
 class SpzError(RuntimeError):
-    pass
+    """A libspz call failed; `code` is its SPZ_E_* return code."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
 
 
 class Config(ctypes.Structure):
@@ -225,12 +229,12 @@ class Solver:
             if self._ctx:
                 L.spz_destroy(self._ctx)
             self._ctx = None
-            raise SpzError(f"spz_create failed ({rc}): {msg}")
+            raise SpzError(f"spz_create failed ({rc}): {msg}", rc)
 
     def _check(self, rc: int, what: str):
         if rc != 0:
             msg = lib().spz_last_error(self._ctx).decode()
-            raise SpzError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}")
+            raise SpzError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}", rc)
 
     def close(self):
         if self._ctx:

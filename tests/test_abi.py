@@ -1,6 +1,8 @@
 """The C ABI without a GPU: libspz.so loads, exports every entry point that
-include/spz.h declares, and its host-only functions (generators, PDB lookup
-geometry, problem-file parsing) agree with the oracle."""
+include/spz.h declares, its host-only functions (generators, PDB lookup
+geometry, problem-file parsing) agree with the oracle, and configurations,
+generator arguments and CLI arguments are checked before a device is
+touched."""
 import os
 import re
 import subprocess
@@ -134,6 +136,16 @@ def test_create_rejects_bad_configs():
     # valid: created on a GPU, SPZ_E_HIP without one
     for kw in (dict(), dict(mode=1), dict(frontier_target=1 << 20)):
         assert create(**kw) != E_ARG, kw
+
+
+def test_solver_rejects_bad_configs():
+    """The Python mirror raises SpzError carrying SPZ_E_ARG (-1) for a bad
+    configuration, before any device is touched."""
+    for kw in (dict(width=3), dict(width=4, heuristic=spz.H_PDB666), dict(mode=5), dict(workgroup_size=100),
+               dict(frontier_target=1 << 21), dict(lb_policy=7), dict(resplit=5000)):
+        with pytest.raises(spz.SpzError) as e:
+            spz.Solver(**kw)
+        assert e.value.code == -1, kw
 
 
 def test_config_layout_matches_c(tmp_path):
