@@ -72,7 +72,8 @@ typedef struct spz_config {
                                 1 = no split; at most 1048576 */
     int32_t max_split_depth; /* cap on that split depth (0 = 24) */
     int32_t workgroup_size;  /* 64, 128 or 256 lanes (0 = 256) */
-    int32_t lanes;           /* DFS lanes in flight (0 = fill the device) */
+    int32_t lanes;           /* DFS lanes in flight (0 = fill the device;
+                                at most 2^24) */
     int32_t round_budget;    /* DFS steps per lane per round (0 = auto:
                                 2048 for Manhattan when the batch has work
                                 for every lane, else 1024) */
@@ -92,7 +93,8 @@ typedef struct spz_config {
                                 (spz_get_samples); training data of the MLP */
     int64_t queue_items;     /* capacity in items of each of the two work-queue
                                 rings (current iterations, lookahead), rounded
-                                up to a power of two (0 = auto) */
+                                up to a power of two (0 = auto; at most
+                                2^30) */
     int32_t record_paths;    /* 1: record one optimal solution path per instance
                                 (spz_get_paths); needs max_split_depth <= 32 */
     int32_t lookahead;       /* 1 (default): while lanes are idle, start an

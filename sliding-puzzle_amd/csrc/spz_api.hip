@@ -333,6 +333,9 @@ static int normalise_config(const spz_config *in, spz_config *c, int *wg) {
     *c = *in;
     if (c->mode != SPZ_MODE_ALL && c->mode != SPZ_MODE_FIRST) return SPZ_E_ARG;
     if (c->frontier_target > kMaxFrontierTarget) return SPZ_E_ARG;
+    // sizes whose buffers are computed in size_t: 2^30 queue items (32 GiB a
+    // ring) and 2^24 lanes keep every product far from wrapping
+    if (c->queue_items > (1ll << 30) || c->lanes > (1 << 24)) return SPZ_E_ARG;
     *wg = c->workgroup_size ? c->workgroup_size : 256;
     if (*wg != 64 && *wg != 128 && *wg != 256) return SPZ_E_ARG;
     c->workgroup_size = *wg;
