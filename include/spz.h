@@ -76,7 +76,7 @@ typedef struct spz_config {
                                 at most 2^24) */
     int32_t round_budget;    /* DFS steps per lane per round (0 = auto:
                                 2048 for Manhattan when the batch has work
-                                for every lane, else 1024) */
+                                for every lane, else 1024; at most 2^20) */
     int32_t lds_stack_depth; /* DFS stack levels per lane kept in LDS (0 =
                                 80 for width 4, 96 for width 5 Manhattan, 64
                                 for the PDB search; at most 160);

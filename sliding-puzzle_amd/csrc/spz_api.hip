@@ -336,6 +336,8 @@ static int normalise_config(const spz_config *in, spz_config *c, int *wg) {
     // sizes whose buffers are computed in size_t: 2^30 queue items (32 GiB a
     // ring) and 2^24 lanes keep every product far from wrapping
     if (c->queue_items > (1ll << 30) || c->lanes > (1 << 24)) return SPZ_E_ARG;
+    // a round's step budget bounds one launch when its time limit is off
+    if (c->round_budget > (1 << 20)) return SPZ_E_ARG;
     *wg = c->workgroup_size ? c->workgroup_size : 256;
     if (*wg != 64 && *wg != 128 && *wg != 256) return SPZ_E_ARG;
     c->workgroup_size = *wg;

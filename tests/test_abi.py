@@ -132,7 +132,7 @@ def test_create_rejects_bad_configs():
     for kw in (dict(mode=2), dict(mode=-1), dict(frontier_target=(1 << 20) + 1), dict(frontier_target=1 << 30),
                dict(workgroup_size=96), dict(lds_stack_depth=4), dict(lb_policy=9), dict(resplit=-1),
                dict(width=6), dict(heuristic=spz.H_PDB666), dict(queue_items=(1 << 30) + 1),
-               dict(queue_items=(1 << 63) - 1), dict(lanes=(1 << 24) + 1)):
+               dict(queue_items=(1 << 63) - 1), dict(lanes=(1 << 24) + 1), dict(round_budget=(1 << 20) + 1)):
         assert create(**kw) == E_ARG, kw
     # valid: created on a GPU, SPZ_E_HIP without one
     for kw in (dict(), dict(mode=1), dict(frontier_target=1 << 20), dict(queue_items=1 << 30)):
