@@ -198,3 +198,20 @@ def test_cli_argument_errors(tmp_path):
     bad.write_text("1 2 3\n")
     r = subprocess.run([exe, str(bad)], capture_output=True, text=True)
     assert r.returncode == 1 and "not a 4x4 or 5x5 board" in r.stderr
+
+
+def test_generators_match_oracle_on_random_seeds():
+    """spz_gen_walk / spz_gen_random against the oracle's generators over
+    random widths, step counts and 64-bit seeds (the fixed-seed test above
+    covers the benchmark seeds)."""
+    hyp = pytest.importorskip("hypothesis")
+    from hypothesis import given, settings, strategies as st
+
+    @settings(max_examples=300, deadline=None)
+    @given(w=st.integers(2, 5), steps=st.integers(0, 400), seed=st.integers(0, (1 << 64) - 1))
+    def check(w, steps, seed):
+        assert (spz.gen_walk(w, steps, seed) == O.walk(w, steps, seed)).all()
+        r = spz.gen_random(w, seed)
+        assert (r == O.random_solvable(w, seed)).all() and O.is_solvable(r)
+
+    check()
