@@ -16,6 +16,7 @@ there is no CPU fallback: a missing library raises.
 from __future__ import annotations
 
 import ctypes
+import re
 import os
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
@@ -172,15 +173,35 @@ def pdb_geometry():
 
 def read_problem(path: str, width: Optional[int] = None) -> np.ndarray:
     """Parse a reference problem file: N*N whitespace-separated integers,
-    0 = blank (src/24puzzle.cc:72-87).  The width is inferred when omitted."""
-    with open(path) as f:
-        vals = [int(x) for x in f.read().split()]
-    if width is None:
-        width = 5 if len(vals) >= 25 else 4
-    n2 = width * width
-    if len(vals) < n2:
-        raise SpzError(f"{path}: expected {n2} integers, found {len(vals)}")
-    return np.array(vals[:n2], dtype=np.uint8)
+    0 = blank (src/24puzzle.cc:72-87), by the CLI's rules: integers are read
+    up to the first other token; 25 or more make a 5x5 board, 16 or more a
+    4x4 one (the rest is ignored); a given width must match; every cell must
+    lie in 0..N*N-1.  Any other file raises SpzError (SPZ_E_ARG)."""
+    vals = []
+    with open(path, "rb") as f:
+        data = f.read()
+    # as a C++ stream reads 64-bit integers: each read skips whitespace and
+    # takes an optionally signed run of digits; reading stops at the first
+    # read that finds none or overflows ("12-3" gives 12 and -3, "12abc" 12)
+    for tok in data.split():
+        pos, ok = 0, True
+        while pos < len(tok) and len(vals) < 25:
+            m = re.match(rb"[+-]?[0-9]+", tok[pos:])
+            if not m or not -(1 << 63) <= int(m.group()) < (1 << 63):
+                ok = False
+                break
+            vals.append(int(m.group()))
+            pos += m.end()
+        if not ok or len(vals) == 25:
+            break
+    w = 5 if len(vals) >= 25 else 4 if len(vals) >= 16 else 0
+    if w == 0 or (width is not None and width != w):
+        raise SpzError(f"{path}: not a {width or 4}x{width or 4} or 5x5 board "
+                       f"({len(vals)} leading integers)", -1)
+    cells = vals[:w * w]
+    if any(c < 0 or c >= w * w for c in cells):
+        raise SpzError(f"{path}: cells must lie in 0..{w * w - 1}", -1)
+    return np.array(cells, dtype=np.uint8)
 
 
 def format_line(path: str, res: Result) -> str:
