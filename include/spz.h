@@ -202,7 +202,9 @@ int spz_get_pdb(spz_ctx *ctx, int32_t pattern, uint8_t *out);
  * driver src/15puzzle_block_parallel.cu:429-565, for a whole batch in one
  * device-resident search.  out[i] receives instance i's result;
  * iter_counts (may be NULL) receives n*max_iters u64 per-iteration counts,
- * row i = thresholds h0_i, h0_i+2, ... (unused entries zero). */
+ * row i = thresholds h0_i, h0_i+2, ... (unused entries zero).  max_iters in
+ * 1..SPZ_MAX_ITERS, n at most 2^24; SPZ_E_ARG otherwise. */
+#define SPZ_MAX_ITERS 128
 int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n,
                     int32_t max_iters, spz_result *out, uint64_t *iter_counts);
 

@@ -324,6 +324,8 @@ static int configure_lanes(spz_ctx *ctx) {
     return SPZ_OK;
 }
 
+static_assert(kMaxIters == SPZ_MAX_ITERS, "the header's iteration limit is the library's");
+
 // Largest root-split target: a level holds at most 4 x target items per
 // instance, so this keeps that capacity far inside 32-bit counts and the
 // frontier buffer at 128 MB per instance.

@@ -44,6 +44,13 @@ def test_error_codes_match_header():
     assert codes == set(spz.ERRORS)
 
 
+def test_limits_match_header():
+    """The Python mirror's limits are the header's."""
+    text = open(HEADER).read()
+    limit = {k: int(v) for k, v in re.findall(r"#define (SPZ_MAX_ITERS|SPZ_NOALLOC_MAX_BOARDS)\s+(\d+)", text)}
+    assert limit == {"SPZ_MAX_ITERS": spz.MAX_ITERS, "SPZ_NOALLOC_MAX_BOARDS": spz.NOALLOC_MAX_BOARDS}
+
+
 def test_header_compiles_as_c():
     src = '#include "spz.h"\nint main(void){spz_config c; spz_default_config(&c,4,0); return c.width!=4;}\n'
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(REPO, "include"),
