@@ -38,14 +38,15 @@ static void usage() {
                  "                         needs --mlp-params)\n"
                  "  --mlp-params FILE      predictor parameters: spz_mlp_param_count() float32\n"
                  "  --resplit K            re-split frontier nodes above K x the mean estimate\n"
-                 "  --frontier N --wg N --lds-depth N --budget N --max-iters N\n"
+                 "  --frontier N --wg N --lds-depth N --budget N --max-iters N (1..%d)\n"
                  "  --autotune             tune the launch configuration on the files first\n"
                  "  --no-lookahead         run each instance's iterations strictly one at a time\n"
                  "  --dedup                drop duplicate boards from the root split (fewer nodes;\n"
                  "                         not the reference's counts)\n"
                  "  --round-time US        a search round also ends after US microseconds (-1: off;\n"
                  "                         default 4500 for the PDB search; Manhattan 3500 when the\n"
-                 "                         batch has work for every lane, else 700)\n");
+                 "                         batch has work for every lane, else 700)\n",
+                 SPZ_MAX_ITERS);
 }
 
 int main(int argc, char **argv) {
@@ -155,7 +156,7 @@ int main(int argc, char **argv) {
             files.push_back(a);
         }
     }
-    if ((files.empty() && tabdir.empty()) || max_iters < 1) {
+    if ((files.empty() && tabdir.empty()) || max_iters < 1 || max_iters > SPZ_MAX_ITERS) {
         usage();
         return 2;
     }

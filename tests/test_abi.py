@@ -196,7 +196,7 @@ def test_cli_argument_errors(tmp_path):
     # are refused, not read as the defaults
     for args in (["-H", "pbd"], ["-m", "frist"], ["--wg", "abc"], ["--frontier", "12x"],
                  ["--budget", ""], ["--max-iters", "99999999999"], ["--max-iters", "0"],
-                 ["--max-iters", "-3"], ["--round-time"]):
+                 ["--max-iters", "-3"], ["--max-iters", "129"], ["--max-iters", "2147483647"], ["--round-time"]):
         r = subprocess.run([exe, *args, str(p)], capture_output=True, text=True)
         assert r.returncode == 2 and "usage: spz_solve" in r.stderr, args
     r = subprocess.run([exe, "--lb", "learned", str(p)], capture_output=True, text=True)
