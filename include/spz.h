@@ -96,7 +96,8 @@ typedef struct spz_config {
                                 up to a power of two (0 = auto; at most
                                 2^30) */
     int32_t record_paths;    /* 1: record one optimal solution path per instance
-                                (spz_get_paths); needs max_split_depth <= 32 */
+                                (spz_get_paths); needs max_split_depth <= 32
+                                and at most 2^27 queue items a ring */
     int32_t lookahead;       /* 1 (default): while lanes are idle, start an
                                 instance's next iteration beside the one that
                                 is draining; counts and lengths are unchanged

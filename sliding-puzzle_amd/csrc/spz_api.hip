@@ -971,6 +971,9 @@ int spz_solve_batch(spz_ctx *ctx, const uint8_t *boards, int32_t n, int32_t max_
     if (paths) {
         if (ctx->cfg.max_split_depth > 32)
             return fail(ctx, SPZ_E_ARG, "record_paths needs max_split_depth <= 32");
+        // the kernels address queued paths in 32-bit words
+        if (2 * qcap * (size_t)kPathWords > (1ull << 32))
+            return fail(ctx, SPZ_E_ARG, "record_paths needs at most 2^27 queue items a ring (smaller batch)");
         HIPCHK(ctx->qpath.ensure(2 * qcap * kPathWords * 4));
         HIPCHK(ctx->c_ipath.ensure((size_t)kPathWords * lanes * 4));
         HIPCHK(ctx->fpath.ensure(front_items * 8));
