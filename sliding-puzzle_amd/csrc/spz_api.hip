@@ -335,8 +335,9 @@ static int normalise_config(const spz_config *in, spz_config *c, int *wg) {
     *c = *in;
     if (c->mode != SPZ_MODE_ALL && c->mode != SPZ_MODE_FIRST) return SPZ_E_ARG;
     if (c->frontier_target > kMaxFrontierTarget) return SPZ_E_ARG;
-    // sizes whose buffers are computed in size_t: 2^30 queue items (32 GiB a
-    // ring) and 2^24 lanes keep every product far from wrapping
+    // 2^30 queue items (32 GiB a ring) keep the buffer sizes far from
+    // wrapping; 2^24 lanes also keep the search kernel's int offsets into
+    // the lane contexts (at most 10 x lanes) in range
     if (c->queue_items > (1ll << 30) || c->lanes > (1 << 24)) return SPZ_E_ARG;
     // a round's step budget bounds one launch when its time limit is off
     if (c->round_budget > (1 << 20)) return SPZ_E_ARG;
